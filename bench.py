@@ -1,0 +1,252 @@
+#!/usr/bin/env python3
+"""Headline benchmark: SU(2) heatbath + overrelaxation sweeps at 48^4, beta=3.09.
+
+A step is one sweep over the whole lattice: one checkerboard Kennedy-Pendleton
+heatbath sweep plus one overrelaxation sweep (16 kernel launches), counted as
+4*VOL link updates.  The FP32-storage chain is the headline `value`; the
+all-FP64 chain is timed the same way for the FP32/FP64 ratio, and both chains'
+plaquettes are compared over a post-timing series.  The field is resident in
+HBM for the whole run (hot start on the device, no host transfers inside the
+timed region).  Kernel durations come from HIP events on the engine's stream
+recorded around every launch of the timed region.
+
+Also reported (extra keys): spatial APE steps/s, Wilson-loop and charge times,
+and the reference CPU code timed on this host (one core) on one sweep of the
+same 48^4 configuration.
+
+Usage: python bench.py [--gpus 1] [--steps K] [--warmup W]
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = ("link updates/s at 48^4 FP32-store vs FP64, same ⟨plaq⟩ ±1σ; "
+          "APE VJP/fwd time")
+HBM_PEAK_GBS = 8000.0
+# algorithmic (compulsory) bytes per link update: every field element moves once
+# per kernel pass; heatbath reads 7 half-planes and writes 1, overrelaxation
+# also reads the old link
+BYTES_PER_UPDATE = {"heatbath": {1: 128, 0: 256}, "overrelax": {1: 144, 0: 288}}
+APE_BYTES_PER_LINK = {1: 32, 0: 64}
+
+
+def log(msg):
+    print(msg, file=sys.stderr, flush=True)
+
+
+def run_chain(lgt, L, prec, seed, beta, n_or, warmup, steps, nplaq):
+    vol = int(np.prod(L))
+    e = lgt.Engine(L, prec, seed)
+    e.init_field(1)
+    e.update(warmup, n_or, beta)
+    e.sync()
+    e.set_option(lgt.engine.OPT_PROFILE, 1)
+    e.profile_reset()
+    e.sync()
+    t0 = time.perf_counter()
+    e.update(steps, n_or, beta)
+    e.sync()
+    t1 = time.perf_counter()
+    kern = {}
+    for cls, name in enumerate(lgt.engine.KERNEL_CLASSES):
+        ms, n = e.profile_read(cls)
+        if n:
+            kern[name] = {"ms_total": ms, "launches": n, "ms_avg": ms / n}
+    e.set_option(lgt.engine.OPT_PROFILE, 0)
+    plaqs = []
+    for _ in range(nplaq):
+        e.update(1, n_or, beta)
+        plaqs.append(e.plaquette())
+    dt = t1 - t0
+    return e, {
+        "seconds": dt,
+        "ms_per_step": 1e3 * dt / steps,
+        "link_updates_per_s": 4.0 * vol * steps / dt,
+        "kernels": kern,
+        "plaq": plaqs,
+        "device_bytes": e.device_bytes(),
+    }
+
+
+def time_ape(e, prec, L, nsteps=20):
+    vol = int(np.prod(L))
+    e.set_option(1, 1)
+    e.profile_reset()
+    e.sync()
+    t0 = time.perf_counter()
+    e.ape(0, nsteps, 0.25)
+    e.sync()
+    dt = time.perf_counter() - t0
+    ms, n = e.profile_read(4)
+    e.set_option(1, 0)
+    avg = ms / n
+    gbs = 3 * vol * APE_BYTES_PER_LINK[prec] / (avg * 1e-3) / 1e9
+    return {"steps_per_s": nsteps / dt, "kernel_ms_avg": avg, "achieved_GBs": gbs, "frac_hbm": gbs / HBM_PEAK_GBS}
+
+
+def time_observables(e):
+    out = {}
+    t0 = time.perf_counter()
+    R, T = np.meshgrid(np.arange(1, 11), np.arange(1, 17), indexing="ij")
+    e.wilson_loops(R.ravel(), T.ravel())
+    out["wloops_R1-10_T1-16_s"] = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    e.qtop(0)
+    out["qtop_improved_s"] = time.perf_counter() - t0
+    return out
+
+
+def cpu_baseline(e64, L, beta):
+    """The reference's own heatbath sweep (LGT_Code/modules/update/update.c:102-176,
+    compiled from its sources into oracle/_ref) on one core, on the thermalised
+    FP64 configuration of the GPU run."""
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    from oracle_lib import REF_SIZES, RefLib
+    name = {v: k for k, v in REF_SIZES.items()}.get(tuple(L))
+    if name is None or not RefLib.available(name):
+        return None
+    ref = RefLib(name)
+    ref.setup(beta, 1)
+    U = e64.download()
+    ref.set_field(U)
+    del U
+    t0 = time.perf_counter()
+    ref.lib.ref_update(1)
+    dt = time.perf_counter() - t0
+    vol = int(np.prod(L))
+    return {
+        "value": 4.0 * vol / dt,
+        "unit": "link updates/s",
+        "cores": 1,
+        "kind": "reference",
+        "sample": f"1 lexicographic heatbath sweep (gaugefieldUpdate, no overrelaxation) of the thermalised "
+                  f"{'x'.join(map(str, L))} FP64 configuration, {dt:.1f} s on 1 host core",
+    }
+
+
+def load_traffic(kernel, workload):
+    path = os.path.join(ROOT, "profiles", "traffic.json")
+    if not os.path.exists(path):
+        return None
+    with open(path) as f:
+        tab = json.load(f)
+    return tab.get(workload, {}).get(kernel)
+
+
+def main():
+    ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=100)
+    ap.add_argument("--warmup", type=int, default=100)
+    ap.add_argument("--lattice", type=int, nargs=4, default=[48, 48, 48, 48])
+    ap.add_argument("--beta", type=float, default=3.09)
+    ap.add_argument("--n-or", type=int, default=1)
+    ap.add_argument("--seed", type=int, default=7)
+    ap.add_argument("--nplaq", type=int, default=50)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py runs on exactly one GPU; --gpus {args.gpus} is not supported (the engine has no "
+              f"multi-GPU path)", file=sys.stderr)
+        sys.exit(2)
+
+    import masterarbeit_amd as lgt
+    lgt.load_library()
+    L = tuple(args.lattice)
+    vol = int(np.prod(L))
+    lname = "x".join(map(str, L))
+    log(f"[bench] lattice {lname} beta={args.beta} n_or={args.n_or} warmup={args.warmup} steps={args.steps}")
+
+    e32, r32 = run_chain(lgt, L, lgt.FP32, args.seed, args.beta, args.n_or, args.warmup, args.steps, args.nplaq)
+    log(f"[bench] fp32 sweep: {r32['ms_per_step']:.3f} ms/step, {r32['link_updates_per_s']:.4g} link updates/s")
+    ape32 = time_ape(e32, 1, L)
+    log(f"[bench] fp32 spatial APE: {ape32['steps_per_s']:.1f} steps/s")
+    obs32 = time_observables(e32)
+    e32.close()
+
+    e64, r64 = run_chain(lgt, L, lgt.FP64, args.seed, args.beta, args.n_or, args.warmup, args.steps, args.nplaq)
+    log(f"[bench] fp64 sweep: {r64['ms_per_step']:.3f} ms/step, {r64['link_updates_per_s']:.4g} link updates/s")
+    ape64 = time_ape(e64, 0, L)
+    obs64 = time_observables(e64)
+
+    # dominant kernel of the headline chain
+    k = r32["kernels"]
+    dom = max(("heatbath", "overrelax"), key=lambda n: k.get(n, {}).get("ms_total", 0.0))
+    per_launch_bytes = (vol // 2) * BYTES_PER_UPDATE[dom][1]
+    achieved = per_launch_bytes / (k[dom]["ms_avg"] * 1e-3) / 1e9
+    traffic = load_traffic(dom, f"{lname}_fp32")
+
+    p32, p64 = np.array(r32["plaq"]), np.array(r64["plaq"])
+    sigma = float(np.sqrt(0.5 * (p32.var(ddof=1) + p64.var(ddof=1)))) if len(p32) > 1 else None
+    diff = float(p32.mean() - p64.mean()) if len(p32) else None
+
+    base = None
+    if not args.no_cpu_baseline:
+        log("[bench] timing the reference CPU sweep (one core) ...")
+        base = cpu_baseline(e64, L, args.beta)
+    e64.close()
+
+    out = {
+        "metric": METRIC,
+        "value": r32["link_updates_per_s"],
+        "unit": "link updates/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": r32["ms_per_step"],
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {
+            "workload": f"SU(2) pure gauge, {lname} lattice, beta={args.beta}: 1 checkerboard heatbath + "
+                        f"{args.n_or} overrelaxation sweep per step, FP32 link storage with FP64 global sums, "
+                        f"hot start (Haar, seed {args.seed}) thermalised by the warm-up sweeps",
+            "lattice": list(L),
+            "beta": args.beta,
+            "n_or": args.n_or,
+        },
+        "roofline": {
+            "bound": "hbm",
+            "kernel": dom,
+            "achieved": achieved,
+            "peak": HBM_PEAK_GBS,
+            "unit": "GB/s",
+            "frac": achieved / HBM_PEAK_GBS,
+            "traffic": traffic,
+            "algorithmic_bytes_per_launch": per_launch_bytes,
+            "avg_launch_ms": k[dom]["ms_avg"],
+        },
+        "cpu_baseline": base,
+        "fp64": {
+            "link_updates_per_s": r64["link_updates_per_s"],
+            "ms_per_step": r64["ms_per_step"],
+            "kernels": r64["kernels"],
+        },
+        "fp32_over_fp64": r32["link_updates_per_s"] / r64["link_updates_per_s"],
+        "fp32_kernels": r32["kernels"],
+        "plaquette": {
+            "fp32_mean": float(p32.mean()) if len(p32) else None,
+            "fp64_mean": float(p64.mean()) if len(p64) else None,
+            "sigma_per_config": sigma,
+            "diff": diff,
+            "within_1sigma": (abs(diff) <= sigma) if sigma else None,
+            "n": len(p32),
+        },
+        "ape_spatial": {"fp32": ape32, "fp64": ape64},
+        "observables_s": {"fp32": obs32, "fp64": obs64},
+        "device_bytes": {"fp32": r32["device_bytes"], "fp64": r64["device_bytes"]},
+    }
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,176 @@
+/*
+ * lgt_su2.h -- C ABI of the MI355X SU(2) pure-gauge link-field engine.
+ *
+ * One engine = one lattice resident in HBM on the current HIP device, in one
+ * of two precisions (FP64 links, or FP32 links with FP64 global sums), plus
+ * one HIP stream on which every call runs.  All calls return an int status
+ * (LGT_OK or a negative LGT_E* code; lgt_last_error() gives the message);
+ * results are complete and copied to host memory when a call returns.  Host
+ * arrays are plain C arrays; no call takes or returns device pointers.
+ *
+ * Field exchange uses the reference's memory order: VOL*4 quaternions, site
+ * major then direction, each {c0,c1,c2,c3} with U = c0 + i(c1 s1+c2 s2+c3 s3);
+ * site n = x3 + L3*(x2 + L2*(x1 + L1*x0)), x0 = t
+ * (LGT_Code/include/gauge.h:21-24, LGT_Code/modules/admin/init.c:768-868).
+ *
+ * Each entry point cites the reference function whose role it takes.  The
+ * reference keeps its state in globals (pu, neib, runParams, measParams,
+ * LGT_Code/include/headers.h:345-413); here the same state lives in the
+ * engine handle and is passed explicitly.  lgt_compat.h exports the
+ * reference's own names on top of this ABI.
+ */
+#ifndef LGT_SU2_H
+#define LGT_SU2_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct lgt_engine lgt_engine;
+
+/* status codes */
+#define LGT_OK 0
+#define LGT_EINVAL (-1)   /* bad argument (extent, size, mode, loop list) */
+#define LGT_ENOMEM (-2)   /* device or host allocation failed */
+#define LGT_EDEVICE (-3)  /* HIP runtime error */
+
+/* precision */
+#define LGT_FP64 0        /* FP64 links and arithmetic */
+#define LGT_FP32 1        /* FP32 links and arithmetic, FP64 global sums */
+
+/* APE smearing modes (LGT_Code/modules/obs/smearing.c:74-168) */
+#define LGT_APE_SPATIAL 0  /* smearing_APE_spatial: links 1..3, 4 spatial staples */
+#define LGT_APE_ALL 1      /* smearing_APE_all: links 0..3, 6 staples */
+#define LGT_APE_TEMPORAL 2 /* smearing_APE_temporal: link 0, 6 staples */
+
+/* topological charge definitions (LGT_Code/modules/obs/topologicalcharge.c) */
+#define LGT_Q_IMPROVED 0   /* topologicalCharge(): 1x1+2x2+3x3 clover, :164-199 */
+#define LGT_Q_CLOVER 1     /* meas_topologicalcharge(): 1x1 clover, :391-409 */
+
+/* engine options (lgt_set_option) */
+#define LGT_OPT_XCD_REMAP 0  /* 1 (default): XCD-contiguous tile order */
+#define LGT_OPT_PROFILE 1    /* 1: time every launch with HIP events */
+
+/* kernel classes of the launch profile (lgt_profile_read) */
+#define LGT_K_HEATBATH 0
+#define LGT_K_OVERRELAX 1
+#define LGT_K_PROJECT 2
+#define LGT_K_PLAQUETTE 3
+#define LGT_K_APE 4
+#define LGT_K_WLOOP 5
+#define LGT_K_QTOP 6
+#define LGT_K_REDUCE 7
+#define LGT_K_FIELD 8      /* cold/hot start, layout conversion */
+#define LGT_K_COUNT 9
+
+/* ---- life cycle ---------------------------------------------------------- */
+
+/* Allocates the field on the current device.  extents[0] = time; every extent
+ * even and >= 2, extents[3] >= 4 (LGT_Code/include/headers.h:58-68; here
+ * runtime values instead of compile-time LENGT..LENGS3).  The seed keys the
+ * engine's counter-based RNG.  The field starts cold.
+ * Replaces initArrayOfNeighbours + initGaugeField's allocation
+ * (LGT_Code/modules/admin/init.c:741-869). */
+int lgt_create(const int extents[4], int precision, uint64_t seed, lgt_engine **out);
+/* releaseGaugeField (LGT_Code/modules/admin/init.c:922-925) */
+int lgt_destroy(lgt_engine *e);
+
+const char *lgt_strerror(int status);
+/* message of the last failed call on this host thread */
+const char *lgt_last_error(void);
+
+int lgt_set_option(lgt_engine *e, int option, int value);
+int lgt_get_extents(const lgt_engine *e, int extents[4], int *precision);
+/* bytes of device memory held by the engine */
+int64_t lgt_device_bytes(const lgt_engine *e);
+
+/* ---- field --------------------------------------------------------------- */
+
+/* initGaugeField(flag) (LGT_Code/modules/admin/init.c:831-869): 0 = cold
+ * (all links 1), 1 = hot.  The hot start is Haar-uniform on S^3 (the
+ * reference's su2RandomMatrix, random_su3.c:141-155, is not), drawn from the
+ * engine's RNG in FP64 and rounded to the storage precision. */
+int lgt_init_field(lgt_engine *e, int start);
+/* Field exchange in the reference layout (pu, LGT_Code/include/headers.h:402);
+ * FP32 engines round on upload and widen on download.  The conversion of
+ * readConfig/writeConfig (LGT_Code/modules/io/config_IO.c:40-117,537-621)
+ * between file and pu. */
+int lgt_upload(lgt_engine *e, const double *aos);
+int lgt_download(lgt_engine *e, double *aos);
+
+/* ---- update -------------------------------------------------------------- */
+
+/* gaugefieldUpdate(iup, nup, utype=1, stype) (LGT_Code/modules/update/
+ * update.c:102-176): nsweeps sweeps at coupling beta.  Each sweep is one
+ * checkerboard Kennedy-Pendleton heatbath sweep (sub-sweeps mu = 0..3,
+ * parity 0,1, heatbath.c:76-149 per link) followed by n_or overrelaxation
+ * sweeps in the same order (U' = V^+ U^+ V^+, V = A/|A|, links re-projected
+ * as they are stored).  With n_or == 0 the whole field is projected after
+ * every sweep (update.c:166).  Sweep k draws its uniforms from Philox4x32-10
+ * keyed by the seed with counter (cb site, mu|parity<<2, sweep index, draw).
+ * acceptance (nullable) = mean over links and sweeps of 1/trials, as the
+ * reference logs it (update.c:159,170-173). */
+int lgt_update(lgt_engine *e, int nsweeps, int n_or, double beta, double *acceptance);
+/* single sub-sweeps (no projection), for replays of a sweep */
+int lgt_heatbath_subsweep(lgt_engine *e, int mu, int parity, double beta, double *acc_sum);
+int lgt_overrelax_subsweep(lgt_engine *e, int mu, int parity);
+/* project_gfield_to_sun (LGT_Code/modules/admin/sun_vfunc.c:82-97) */
+int lgt_project(lgt_engine *e);
+/* sweep counter of the RNG (advanced by lgt_update; set to resume a chain) */
+int lgt_set_sweep_index(lgt_engine *e, uint32_t sweep);
+uint32_t lgt_get_sweep_index(const lgt_engine *e);
+
+/* ---- observables ---------------------------------------------------------- */
+
+/* plaquette() (LGT_Code/modules/obs/plaquette.c:67-101): mean of
+ * (1/2) Re tr U_p over the 6 VOL plaquettes, FP64 sum. */
+int lgt_plaquette(lgt_engine *e, double *plaq);
+/* gaugeAction() (plaquette.c:104-110): beta * 6 VOL * (1 - plaq) */
+int lgt_gauge_action(lgt_engine *e, double beta, double *action);
+
+/* smearing_APE_spatial / _all / _temporal(ns, sp, u) (LGT_Code/modules/obs/
+ * smearing.c:74-168): nsteps Jacobi steps U <- proj[U + sp * staples].
+ * sp is the reference's raw parameter; the usual alpha form
+ * U <- proj[(1-alpha) U + alpha/n S] is sp = alpha / (n (1 - alpha)) with
+ * n = 4 (spatial) or 6 (all, temporal). */
+int lgt_ape(lgt_engine *e, int mode, int nsteps, double sp);
+
+/* measureWilsonLoop(w) (LGT_Code/modules/obs/wilson.c:268-324) with the loop
+ * list given explicitly (measParams.rExtents/tExtents, wilson.c:274-285):
+ * w[i] = mean over sites x and spatial d of (1/2) Re tr W_{R[i] x T[i]}(x; t, d).
+ * Requires 1 <= T[i] <= extents[0]-1 (wilson.c:290-291) and R[i] >= 1. */
+int lgt_wilson_loops(lgt_engine *e, int nloop, const int *R, const int *T, double *w);
+
+/* topologicalCharge() / meas_topologicalcharge(q) (LGT_Code/modules/obs/
+ * topologicalcharge.c:164-199, 391-409).  q: the charge; abs_sum (nullable):
+ * the same sum over |q(x)|, the scale of its rounding error; density
+ * (nullable): q(x) per site in lexicographic order, VOL doubles. */
+int lgt_qtop(lgt_engine *e, int kind, double *q, double *abs_sum, double *density);
+
+/* ---- per-link harnesses (injected inputs, FP64) ---------------------------
+ * The per-link heatbath of localHeatbathUpdate after the staple
+ * (LGT_Code/modules/update/heatbath.c:133-146), fed with given uniforms: link i
+ * has staple staples[4i..4i+3] and consumes uniforms[offsets[i] ..
+ * offsets[i]+lens[i]) in the reference's order (4 per trial, then 2; 3 on the
+ * degenerate branch).  counts[i] = trials (0: degenerate, -1: ran out). */
+int lgt_heatbath_harness(int n, const double *staples, double beta, const double *uniforms,
+                         const int *offsets, const int *lens, double *links, int *counts);
+/* The overrelaxation map on given staples and links (no reference
+ * counterpart; the update above adds it to the heatbath). */
+int lgt_overrelax_harness(int n, const double *staples, const double *links_in, double *links_out);
+
+/* ---- timing --------------------------------------------------------------- */
+/* With LGT_OPT_PROFILE on, every launch is bracketed by HIP events on the
+ * engine's stream; this returns the summed kernel time and launch count of
+ * one kernel class since the last reset. */
+int lgt_profile_read(lgt_engine *e, int kernel_class, double *ms_total, int64_t *launches);
+int lgt_profile_reset(lgt_engine *e);
+/* waits for the engine's stream */
+int lgt_sync(lgt_engine *e);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,758 @@
+// kernels.hpp -- HIP kernels of the SU(2) link-field engine, templated on the
+// storage/arithmetic type T (double: all-FP64, compiled without FMA
+// contraction; float: FP32 links and arithmetic, FP64 accumulations).
+//
+// Every kernel processes one or both checkerboard planes with one thread per
+// link or site; blocks are 256 threads (four 64-wide waves).  Reductions are
+// deterministic: per-thread double-double sums, wave shuffles, a fixed-order
+// combination of the waves of a block, one partial per block, and a
+// fixed-order final pass.
+#pragma once
+
+#include "su2_device.hpp"
+
+namespace lgt {
+
+constexpr int BLOCK = 256;
+constexpr int NWAVES = BLOCK / 64;
+constexpr double PI_D = 3.14159265358979323846;   // = 2 asin(1), LGT_Code/modules/admin/init.c:63
+
+// Workgroups are dispatched round-robin over the 8 XCDs of the chip; with
+// remap on, XCD k processes the k-th contiguous eighth of the grid so that
+// neighbouring tiles share one L2.
+__device__ __forceinline__ uint32_t tile_id(int remap)
+{
+    uint32_t b = blockIdx.x, nb = gridDim.x;
+    if (remap && (nb & 7u) == 0) b = (b & 7u) * (nb >> 3) + (b >> 3);
+    return b;
+}
+
+// ---------------------------------------------------------------------------
+// reductions
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ DD wave_sum_dd(DD v)
+{
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) {
+        DD o;
+        o.hi = __shfl_xor(v.hi, off);
+        o.lo = __shfl_xor(v.lo, off);
+        v = dd_add(v, o);
+    }
+    return v;
+}
+
+// block-wide double-double sum, valid in thread 0
+__device__ __forceinline__ DD block_sum_dd(DD v)
+{
+    __shared__ DD ws[NWAVES];
+    v = wave_sum_dd(v);
+    int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+    if (lane == 0) ws[wid] = v;
+    __syncthreads();
+    DD s = DD{0.0, 0.0};
+    if (threadIdx.x == 0)
+        for (int w = 0; w < NWAVES; w++) s = dd_add(s, ws[w]);
+    __syncthreads();
+    return s;
+}
+
+// out[row] = fixed-order dd sum of in[row*ncols .. row*ncols+ncols)
+static __global__ void __launch_bounds__(BLOCK) k_reduce_rows(const DD* __restrict__ in, int ncols, DD* __restrict__ out)
+{
+    const DD* row = in + (size_t)blockIdx.x * ncols;
+    DD a = DD{0.0, 0.0};
+    for (int c = threadIdx.x; c < ncols; c += BLOCK) a = dd_add(a, row[c]);
+    a = block_sum_dd(a);
+    if (threadIdx.x == 0) out[blockIdx.x] = a;
+}
+
+// ---------------------------------------------------------------------------
+// field initialisation and host-layout conversion
+// ---------------------------------------------------------------------------
+// grid: x over cb sites, y = mu*2 + parity
+template <typename T>
+__global__ void __launch_bounds__(BLOCK) k_cold(Planes<T> P, Geo g)
+{
+    uint32_t i = blockIdx.x * BLOCK + threadIdx.x;
+    int mu = blockIdx.y >> 1, p = blockIdx.y & 1;
+    if (i < g.vh) qstore<T>(P.p[mu][p], i, qunit<T>());
+}
+
+// Haar-uniform links (Shoemake's method) from the link's Philox stream,
+// always evaluated in FP64 so both storage precisions start from the same
+// configuration.
+template <typename T>
+__global__ void __launch_bounds__(BLOCK) k_hot(Planes<T> P, Geo g, uint64_t seed)
+{
+    uint32_t i = blockIdx.x * BLOCK + threadIdx.x;
+    int mu = blockIdx.y >> 1, p = blockIdx.y & 1;
+    if (i >= g.vh) return;
+    RngSrc64 src(seed, i, (uint32_t)(mu | (p << 2) | (KIND_HOT << 3)), 0u);
+    double r1 = src.next(), r2 = src.next(), r3 = src.next();
+    double s1 = sqrt(1.0 - r1), s2 = sqrt(r1);
+    double t1 = 2.0 * PI_D * r2, t2 = 2.0 * PI_D * r3;
+    Q<T> q;
+    q.c0 = (T)(s2 * cos(t2));
+    q.c1 = (T)(s1 * sin(t1));
+    q.c2 = (T)(s1 * cos(t1));
+    q.c3 = (T)(s2 * sin(t2));
+    qstore<T>(P.p[mu][p], i, q);
+}
+
+// lexicographic site of cb index i, parity p
+__device__ __forceinline__ uint64_t lex_index(const Geo& g, const Site& s)
+{
+    return (((uint64_t)s.c[0] * g.L[1] + s.c[1]) * g.L[2] + s.c[2]) * g.L[3] + s.c[3];
+}
+
+// reference AoS FP64 [VOL][4][4] -> planes
+template <typename T>
+__global__ void __launch_bounds__(BLOCK) k_from_aos(Planes<T> P, Geo g, const double* __restrict__ aos)
+{
+    uint32_t i = blockIdx.x * BLOCK + threadIdx.x;
+    int mu = blockIdx.y >> 1, p = blockIdx.y & 1;
+    if (i >= g.vh) return;
+    uint64_t n = lex_index(g, cb_site(g, i, p));
+    const double* a = aos + 16 * n + 4 * mu;
+    qstore<T>(P.p[mu][p], i, Q<T>{(T)a[0], (T)a[1], (T)a[2], (T)a[3]});
+}
+
+template <typename T>
+__global__ void __launch_bounds__(BLOCK) k_to_aos(Planes<T> P, Geo g, double* __restrict__ aos)
+{
+    uint32_t i = blockIdx.x * BLOCK + threadIdx.x;
+    int mu = blockIdx.y >> 1, p = blockIdx.y & 1;
+    if (i >= g.vh) return;
+    uint64_t n = lex_index(g, cb_site(g, i, p));
+    Q<T> q = qload<T>(P.p[mu][p], i);
+    double* a = aos + 16 * n + 4 * mu;
+    a[0] = (double)q.c0; a[1] = (double)q.c1; a[2] = (double)q.c2; a[3] = (double)q.c3;
+}
+
+template <typename T>
+__global__ void __launch_bounds__(BLOCK) k_project(Planes<T> P, Geo g)
+{
+    uint32_t i = blockIdx.x * BLOCK + threadIdx.x;
+    int mu = blockIdx.y >> 1, p = blockIdx.y & 1;
+    if (i >= g.vh) return;
+    qstore<T>(P.p[mu][p], i, qproject(qload<T>(P.p[mu][p], i)));
+}
+
+// ---------------------------------------------------------------------------
+// staple of the update (LGT_Code/modules/update/metro.c:34-63)
+//   A = sum_{k != mu} [U_k(n+mu) U_mu(n+k)^+ U_k(n)^+ + U_k(n-k+mu)^+ U_mu(n-k)^+ U_k(n-k)]
+// Site s has parity p; n+mu, n+k, n-k have parity 1-p, n-k+mu has parity p.
+// ---------------------------------------------------------------------------
+template <typename T>
+__device__ __forceinline__ Q<T> lk(const Planes<T>& P, const Geo& g, int mu, int par, const Site& s)
+{
+    return qload<T>(P.p[mu][par], cb_index(g, s));
+}
+
+template <typename T, int MU>
+__device__ __forceinline__ Q<T> update_staple(const Planes<T>& P, const Geo& g, const Site& s, int p)
+{
+    const int q = p ^ 1;
+    Q<T> acc = qzero<T>();
+    Site smu = shift(g, s, MU, 1);
+#pragma unroll
+    for (int k = 0; k < 4; k++) {
+        if (k == MU) continue;
+        Site sk = shift(g, s, k, 1);
+        Q<T> un1 = qmul_dag(lk(P, g, k, q, smu), lk(P, g, MU, q, sk));
+        Q<T> un0 = qmul_dag(un1, lk(P, g, k, p, s));
+        Q<T> un2 = qadd(acc, un0);
+        Site skm = shift(g, s, k, -1);
+        Site skmmu = shift(g, skm, MU, 1);
+        un0 = qdag(lk(P, g, k, p, skmmu));
+        un1 = qmul_dag(un0, lk(P, g, MU, q, skm));
+        un0 = qmul(un1, lk(P, g, k, q, skm));
+        acc = qadd(un2, un0);
+    }
+    return acc;
+}
+
+// ---------------------------------------------------------------------------
+// Kennedy-Pendleton heatbath (LGT_Code/modules/update/heatbath.c:76-149)
+// ---------------------------------------------------------------------------
+// FP64: the reference formula term by term (pow(x,2) evaluated as x*x).
+// The trial loop is bounded: a source that runs dry or KP_MAX_TRIALS
+// rejections (only for alpha -> 0) end it with accepted = false.
+constexpr int KP_MAX_TRIALS = 10000;
+
+template <class Src>
+__device__ __forceinline__ int kp_draw64(double alpha, Src& src, Q<double>& X, bool& accepted)
+{
+    int count = 0;
+    double delta, u3;
+    do {
+        double u0 = src.next(), u1 = src.next(), u2 = src.next();
+        u3 = src.next();
+        double X1 = -log(1.0 - u0) / alpha;
+        double X2 = -log(1.0 - u1) / alpha;
+        double cc = cos(2.0 * PI_D * u2);
+        double C = cc * cc;
+        double A = X1 * C;
+        delta = X2 + A;
+        count++;
+        accepted = !(u3 * u3 > 1.0 - (0.5 * delta));
+    } while (!accepted && count < KP_MAX_TRIALS && !src.dry());
+    if (!accepted) return count;
+    double x0 = 1.0 - delta;
+    double a0 = src.next(), a1 = src.next();
+    double phi = 2.0 * PI_D * a0;
+    double theta = acos(1.0 - (2.0 * a1));
+    double length = sqrt(1.0 - x0 * x0);
+    X.c0 = x0;
+    X.c1 = length * sin(theta) * cos(phi);
+    X.c2 = length * sin(theta) * sin(phi);
+    X.c3 = length * cos(theta);
+    return count;
+}
+
+// su2RandomMatrix (LGT_Code/modules/random/random_su3.c:141-155)
+template <class Src>
+__device__ __forceinline__ Q<double> random_su2_64(Src& src)
+{
+    double r0 = src.next(), r1 = src.next(), r2 = src.next();
+    const double twopi = 2. * PI_D;
+    Q<double> u;
+    u.c0 = 1 - 2. * r0;
+    u.c1 = 1 - 2. * r1;
+    double zw = sqrt(1. - u.c1 * u.c1);
+    u.c2 = zw * cos(twopi * r2);
+    u.c3 = zw * sin(twopi * r2);
+    zw = sqrt(1. - u.c0 * u.c0);
+    u.c1 *= zw; u.c2 *= zw; u.c3 *= zw;
+    return u;
+}
+
+// localHeatbathUpdate for a given staple (heatbath.c:124-149); returns 1/count.
+// Unold is kept when the bounded trial loop ends without acceptance.
+template <class Src>
+__device__ __forceinline__ double heatbath_link64(Q<double> A, double beta, Src& src, const Q<double>& Unold,
+                                                  Q<double>& Unew)
+{
+    double sqrt_det = qdet_sqrt(A);
+    if (sqrt_det <= __DBL_EPSILON__) {
+        Unew = random_su2_64(src);
+        return 0.0;
+    }
+    A = qdiv(A, sqrt_det);
+    double alpha = sqrt_det * beta;
+    Q<double> X;
+    bool ok;
+    int count = kp_draw64(alpha, src, X, ok);
+    Unew = ok ? qmul_dag(X, A) : Unold;
+    return 1.0 / count;
+}
+
+// FP32: one Philox call per trial (four 24-bit uniforms), hardware
+// transcendentals (v_log_f32 = log2, v_cos_f32/v_sin_f32 take revolutions),
+// 1 - x0^2 as delta(2 - delta) and sin(acos(c)) as 2 sqrt(a(1-a)).
+__device__ __forceinline__ double heatbath_link32(Q<float> A, float beta, LinkRng& rng, const Q<float>& Unold,
+                                                  Q<float>& Unew)
+{
+    float n2 = A.c0 * A.c0 + A.c1 * A.c1 + A.c2 * A.c2 + A.c3 * A.c3;
+    if (n2 <= 1e-30f) {
+        U32x4 w = rng.next();
+        float r0 = u24(w.x), r1 = u24(w.y), r2 = u24(w.z);
+        Q<float> u;
+        u.c0 = 1.0f - 2.0f * r0;
+        u.c1 = 1.0f - 2.0f * r1;
+        float zw = sqrtf(fmaxf(0.0f, 1.0f - u.c1 * u.c1));
+        u.c2 = zw * __builtin_amdgcn_cosf(r2);
+        u.c3 = zw * __builtin_amdgcn_sinf(r2);
+        zw = sqrtf(fmaxf(0.0f, 1.0f - u.c0 * u.c0));
+        u.c1 *= zw; u.c2 *= zw; u.c3 *= zw;
+        Unew = u;
+        return 0.0;
+    }
+    float rn = rsqrtf(n2);
+    float a = n2 * rn;                      // |A|
+    Q<float> V = qscale(A, rn);
+    const float kln2 = 0.69314718055994531f;
+    float s = -kln2 / (a * beta);           // -1/alpha * ln 2
+    int count = 0;
+    float delta;
+    bool accepted;
+    do {
+        U32x4 w = rng.next();
+        float l0 = __builtin_amdgcn_logf(1.0f - u24(w.x));   // log2
+        float l1 = __builtin_amdgcn_logf(1.0f - u24(w.y));
+        float c = __builtin_amdgcn_cosf(u24(w.z));
+        float u3 = u24(w.w);
+        delta = s * (l1 + l0 * (c * c));
+        count++;
+        accepted = !(u3 * u3 > 1.0f - 0.5f * delta);
+    } while (!accepted && count < KP_MAX_TRIALS);
+    if (!accepted) {
+        Unew = Unold;
+        return 1.0 / count;
+    }
+    U32x4 w = rng.next();
+    float a0 = u24(w.x), a1 = u24(w.y);
+    float ct = 1.0f - 2.0f * a1;
+    float st = 2.0f * sqrtf(a1 * (1.0f - a1));
+    float len = sqrtf(fmaxf(0.0f, delta * (2.0f - delta)));
+    Q<float> X;
+    X.c0 = 1.0f - delta;
+    X.c1 = len * st * __builtin_amdgcn_cosf(a0);
+    X.c2 = len * st * __builtin_amdgcn_sinf(a0);
+    X.c3 = len * ct;
+    Unew = qmul_dag(X, V);
+    return 1.0 / count;
+}
+
+template <typename T> struct HB;
+template <> struct HB<double> {
+    __device__ static __forceinline__ double update(const Q<double>& A, double beta, uint64_t seed, uint32_t i,
+                                                    uint32_t tag, uint32_t sweep, const Q<double>& Uo,
+                                                    Q<double>& Un)
+    {
+        RngSrc64 src(seed, i, tag, sweep);
+        return heatbath_link64(A, beta, src, Uo, Un);
+    }
+};
+template <> struct HB<float> {
+    __device__ static __forceinline__ double update(const Q<float>& A, float beta, uint64_t seed, uint32_t i,
+                                                    uint32_t tag, uint32_t sweep, const Q<float>& Uo, Q<float>& Un)
+    {
+        LinkRng rng(seed, i, tag, sweep);
+        return heatbath_link32(A, beta, rng, Uo, Un);
+    }
+};
+
+// heatbath sub-sweep of direction MU, parity p.  acc_partial[tile] receives
+// the block's sum of 1/count (nullable).
+template <typename T, int MU>
+__global__ void __launch_bounds__(BLOCK) k_heatbath(Planes<T> P, Geo g, int p, T beta, uint64_t seed, uint32_t sweep,
+                                                    int remap, DD* __restrict__ acc_partial)
+{
+    uint32_t tile = tile_id(remap);
+    uint32_t i = tile * BLOCK + threadIdx.x;
+    double acc = 0.0;
+    if (i < g.vh) {
+        Site s = cb_site(g, i, p);
+        Q<T> A = update_staple<T, MU>(P, g, s, p);
+        Q<T> Uo = qload<T>(P.p[MU][p], i), Un;
+        acc = HB<T>::update(A, beta, seed, i, (uint32_t)(MU | (p << 2) | (KIND_HB << 3)), sweep, Uo, Un);
+        qstore<T>(P.p[MU][p], i, Un);
+    }
+    if (acc_partial) {
+        DD s = block_sum_dd(DD{acc, 0.0});
+        if (threadIdx.x == 0) acc_partial[tile] = s;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// overrelaxation: U' = V^+ U^+ V^+, V = A/|A|, projected on store
+// ---------------------------------------------------------------------------
+template <typename T>
+__device__ __forceinline__ Q<T> overrelax_link(const Q<T>& A, const Q<T>& U);
+
+template <> __device__ __forceinline__ Q<double> overrelax_link(const Q<double>& A, const Q<double>& U)
+{
+    double a = qdet_sqrt(A);
+    if (a <= __DBL_EPSILON__) return U;
+    Q<double> V = qdiv(A, a);
+    Q<double> W = qmul(qdag(V), qdag(U));
+    return qproject(qmul_dag(W, V));
+}
+
+template <> __device__ __forceinline__ Q<float> overrelax_link(const Q<float>& A, const Q<float>& U)
+{
+    float n2 = A.c0 * A.c0 + A.c1 * A.c1 + A.c2 * A.c2 + A.c3 * A.c3;
+    if (n2 <= 1e-30f) return qproject(U);
+    // V^+ U^+ V^+ = (V U V)^+; |V|=1 is not needed: the projection removes |A|^2
+    Q<float> W = qmul(qdag(A), qdag(U));
+    return qproject(qmul_dag(W, A));
+}
+
+template <typename T, int MU>
+__global__ void __launch_bounds__(BLOCK) k_overrelax(Planes<T> P, Geo g, int p, int remap)
+{
+    uint32_t i = tile_id(remap) * BLOCK + threadIdx.x;
+    if (i >= g.vh) return;
+    Site s = cb_site(g, i, p);
+    Q<T> A = update_staple<T, MU>(P, g, s, p);
+    Q<T> U = qload<T>(P.p[MU][p], i);
+    qstore<T>(P.p[MU][p], i, overrelax_link(A, U));
+}
+
+// ---------------------------------------------------------------------------
+// per-link harnesses with injected inputs (FP64)
+// ---------------------------------------------------------------------------
+static __global__ void __launch_bounds__(BLOCK) k_hb_harness(int n, const double* __restrict__ staples, double beta,
+                                                      const double* __restrict__ uniforms,
+                                                      const int* __restrict__ offsets, const int* __restrict__ lens,
+                                                      double* __restrict__ links, int* __restrict__ counts)
+{
+    int i = blockIdx.x * BLOCK + threadIdx.x;
+    if (i >= n) return;
+    BufSrc src{uniforms + offsets[i], lens[i], 0};
+    Q<double> A{staples[4 * i], staples[4 * i + 1], staples[4 * i + 2], staples[4 * i + 3]};
+    Q<double> U, U0 = qzero<double>();
+    double r = heatbath_link64(A, beta, src, U0, U);
+    links[4 * i] = U.c0; links[4 * i + 1] = U.c1; links[4 * i + 2] = U.c2; links[4 * i + 3] = U.c3;
+    counts[i] = (r > 0.0) ? (int)(1.0 / r + 0.5) : 0;
+    if (src.pos > src.n) counts[i] = -1;
+}
+
+static __global__ void __launch_bounds__(BLOCK) k_or_harness(int n, const double* __restrict__ staples,
+                                                      const double* __restrict__ links_in, double* __restrict__ links)
+{
+    int i = blockIdx.x * BLOCK + threadIdx.x;
+    if (i >= n) return;
+    Q<double> A{staples[4 * i], staples[4 * i + 1], staples[4 * i + 2], staples[4 * i + 3]};
+    Q<double> U{links_in[4 * i], links_in[4 * i + 1], links_in[4 * i + 2], links_in[4 * i + 3]};
+    Q<double> R = overrelax_link(A, U);
+    links[4 * i] = R.c0; links[4 * i + 1] = R.c1; links[4 * i + 2] = R.c2; links[4 * i + 3] = R.c3;
+}
+
+// ---------------------------------------------------------------------------
+// plaquette (LGT_Code/modules/obs/plaquette.c:67-101)
+// ---------------------------------------------------------------------------
+template <typename T>
+__device__ __forceinline__ T plaq_term(const Planes<T>& P, const Geo& g, const Site& s, int jj, int kk)
+{
+    Q<T> un0 = link(P, g, jj, s);
+    Q<T> un1 = link(P, g, kk, shift(g, s, jj, 1));
+    Q<T> un2 = qmul(un0, un1);
+    un0 = qdag(link(P, g, jj, shift(g, s, kk, 1)));
+    un1 = qmul(un2, un0);
+    un2 = qdag(link(P, g, kk, s));
+    un0 = qmul(un1, un2);
+    return T(2) * un0.c0;
+}
+
+// grid: x over cb sites, y = parity; partial[y*gridDim.x + tile]
+template <typename T>
+__global__ void __launch_bounds__(BLOCK) k_plaquette(Planes<T> P, Geo g, int remap, DD* __restrict__ partial)
+{
+    uint32_t tile = tile_id(remap);
+    uint32_t i = tile * BLOCK + threadIdx.x;
+    int p = blockIdx.y;
+    DD acc = DD{0.0, 0.0};
+    if (i < g.vh) {
+        Site s = cb_site(g, i, p);
+#pragma unroll
+        for (int jj = 0; jj < 3; jj++)
+#pragma unroll
+            for (int kk = jj + 1; kk < 4; kk++) acc = dd_add(acc, (double)plaq_term(P, g, s, jj, kk));
+    }
+    acc = block_sum_dd(acc);
+    if (threadIdx.x == 0) partial[blockIdx.y * gridDim.x + tile] = acc;
+}
+
+// ---------------------------------------------------------------------------
+// APE smearing (LGT_Code/modules/obs/smearing.c:42-168), Jacobi form
+//   U'_mu(n) = proj[ Z_mu(n) + sp * S_mu(n) ],  S from the step-start field Z
+//   S = sum_{k != mu, dt} [Z_k(n) Z_mu(n+k) Z_k(n+mu)^+ + Z_k(n-k)^+ Z_mu(n-k) Z_k(n-k+mu)]
+// MODE 0: spatial (mu = 1..3, dt = 0), 1: all (mu = 0..3, dt = mu),
+// 2: temporal (mu = 0, dt = 0).
+// ---------------------------------------------------------------------------
+template <typename T>
+__device__ __forceinline__ Q<T> ape_staple(const Planes<T>& Z, const Geo& g, const Site& s, int mu, int dt)
+{
+    Q<T> acc = qzero<T>();
+    Site smu = shift(g, s, mu, 1);
+#pragma unroll
+    for (int k = 0; k < 4; k++) {
+        if (k == mu || k == dt) continue;
+        Site sk = shift(g, s, k, 1);
+        Q<T> un1 = qmul(link(Z, g, k, s), link(Z, g, mu, sk));
+        Q<T> un0 = qmul_dag(un1, link(Z, g, k, smu));
+        Q<T> un2 = qadd(acc, un0);
+        Site skm = shift(g, s, k, -1);
+        un0 = qdag(link(Z, g, k, skm));
+        un1 = qmul(un0, link(Z, g, mu, skm));
+        un0 = qmul(un1, link(Z, g, k, shift(g, skm, mu, 1)));
+        acc = qadd(un2, un0);
+    }
+    return acc;
+}
+
+// grid: x over cb sites, y = (mu - mu0)*2 + parity
+template <typename T, int MODE>
+__global__ void __launch_bounds__(BLOCK) k_ape(Planes<T> Z, Planes<T> O, Geo g, T sp, int remap)
+{
+    uint32_t i = tile_id(remap) * BLOCK + threadIdx.x;
+    if (i >= g.vh) return;
+    int p = blockIdx.y & 1;
+    int mu = (MODE == 0 ? 1 : 0) + (blockIdx.y >> 1);
+    int dt = (MODE == 1) ? mu : 0;
+    Site s = cb_site(g, i, p);
+    Q<T> S = ape_staple(Z, g, s, mu, dt);
+    S = qscale(S, sp);
+    Q<T> U = qadd(qload<T>(Z.p[mu][p], i), S);
+    qstore<T>(O.p[mu][p], i, qproject(U));
+}
+
+// ---------------------------------------------------------------------------
+// Wilson loops W(R,T) in the (t, d) planes, d = 1..3 (LGT_Code/modules/obs/
+// wilson.c:21-56,268-324), fused over all T of one R:
+//   W(x) = Re tr[ L_T(x) S_R(x+T t) L_T(x+R d)^+ S_R(x)^+ ]
+//        = 2 < L_T(x) S_R(x+T t), S_R(x) L_T(x+R d) >
+// with temporal lines L_T grown one link per step and the spatial line at
+// x+T t rebuilt per step.  Results per (group, d, parity, tile) are reduced in
+// double-double; every wave adds its shuffle-reduced value to its own LDS slot.
+// ---------------------------------------------------------------------------
+struct WGroup { int R, Tmax, slot_off, pad; };
+constexpr int WMAXT = 128;
+
+template <typename T>
+__device__ __forceinline__ Q<T> spatial_line(const Planes<T>& P, const Geo& g, Site y, int d, int R)
+{
+    Q<T> S = link(P, g, d, y);
+    for (int j = 1; j < R; j++) {
+        y = shift(g, y, d, 1);
+        S = qmul(S, link(P, g, d, y));
+    }
+    return S;
+}
+
+// grid: x over cb sites, y = parity, z = group*3 + (d-1)
+template <typename T>
+__global__ void __launch_bounds__(BLOCK) k_wloops(Planes<T> P, Geo g, const WGroup* __restrict__ groups,
+                                                  const int* __restrict__ slots, int remap, DD* __restrict__ partial,
+                                                  int ncols)
+{
+    __shared__ DD wacc[NWAVES][WMAXT];
+    uint32_t tile = tile_id(remap);
+    uint32_t i = tile * BLOCK + threadIdx.x;
+    const int p = blockIdx.y;
+    const WGroup grp = groups[blockIdx.z / 3];
+    const int d = 1 + (int)(blockIdx.z % 3);
+    const int R = grp.R;
+    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+    for (int t = threadIdx.x; t < NWAVES * WMAXT; t += BLOCK) wacc[t / WMAXT][t % WMAXT] = DD{0.0, 0.0};
+    __syncthreads();
+
+    const bool valid = i < g.vh;
+    Site s = cb_site(g, valid ? i : 0u, p);
+    Site sR = shift(g, s, d, R % g.L[d]);
+    Q<T> SR = spatial_line(P, g, s, d, R);
+    Q<T> Lx = link(P, g, 0, s);
+    Q<T> LxR = link(P, g, 0, sR);
+    Site top = s, topR = sR;
+    for (int Tt = 1; Tt <= grp.Tmax; Tt++) {
+        if (Tt > 1) {
+            Lx = qmul(Lx, link(P, g, 0, top));
+            LxR = qmul(LxR, link(P, g, 0, topR));
+        }
+        top = shift(g, top, 0, 1);
+        topR = shift(g, topR, 0, 1);
+        int slot = slots[grp.slot_off + Tt - 1];
+        if (slot >= 0) {
+            Q<T> St = spatial_line(P, g, top, d, R);
+            T w = T(2) * qdot(qmul(Lx, St), qmul(SR, LxR));
+            DD v = wave_sum_dd(DD{valid ? (double)w : 0.0, 0.0});
+            if (lane == 0) wacc[wid][Tt - 1] = dd_add(wacc[wid][Tt - 1], v);
+        }
+    }
+    __syncthreads();
+    for (int t = threadIdx.x; t < grp.Tmax; t += BLOCK) {
+        int slot = slots[grp.slot_off + t];
+        if (slot < 0) continue;
+        DD a = wacc[0][t];
+        for (int w = 1; w < NWAVES; w++) a = dd_add(a, wacc[w][t]);
+        int col = ((d - 1) * 2 + p) * gridDim.x + tile;
+        partial[(size_t)slot * ncols + col] = a;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// improved clover charge (LGT_Code/modules/obs/topologicalcharge.c:18-199)
+// ---------------------------------------------------------------------------
+// computeLine: l steps along +dir (t = t U) or -dir (t = t U^+), starting from
+// the identity when `first` (1*U = U exactly).
+template <typename T>
+__device__ __forceinline__ void walk(const Planes<T>& P, const Geo& g, Q<T>& t, Site& n, int dir, int l, bool back,
+                                     bool& first)
+{
+    for (int i = 0; i < l; i++) {
+        if (!back) {
+            Q<T> u = link(P, g, dir, n);
+            if (first) t = u;
+            else t = qmul(t, u);
+            n = shift(g, n, dir, 1);
+        } else {
+            n = shift(g, n, dir, -1);
+            Q<T> u = link(P, g, dir, n);
+            if (first) t = qdag(u);
+            else t = qmul_dag(t, u);
+        }
+        first = false;
+    }
+}
+
+// computeCloverAverages(u, x, d1, d2, m, m): four m x m leaves / 4
+template <typename T>
+__device__ __forceinline__ Q<T> clover_average(const Planes<T>& P, const Geo& g, const Site& x, int d1, int d2, int m)
+{
+    Q<T> sum, t;
+    Site n = x;
+    bool f;
+    // +d1,+d2
+    f = true;
+    walk(P, g, t, n, d1, m, false, f); walk(P, g, t, n, d2, m, false, f);
+    walk(P, g, t, n, d1, m, true, f);  walk(P, g, t, n, d2, m, true, f);
+    sum = t;
+    // +d1,-d2
+    f = true;
+    walk(P, g, t, n, d2, m, true, f);  walk(P, g, t, n, d1, m, false, f);
+    walk(P, g, t, n, d2, m, false, f); walk(P, g, t, n, d1, m, true, f);
+    sum = qadd(sum, t);
+    // -d1,+d2
+    f = true;
+    walk(P, g, t, n, d2, m, false, f); walk(P, g, t, n, d1, m, true, f);
+    walk(P, g, t, n, d2, m, true, f);  walk(P, g, t, n, d1, m, false, f);
+    sum = qadd(sum, t);
+    // -d1,-d2
+    f = true;
+    walk(P, g, t, n, d1, m, true, f);  walk(P, g, t, n, d2, m, true, f);
+    walk(P, g, t, n, d1, m, false, f); walk(P, g, t, n, d2, m, false, f);
+    sum = qadd(sum, t);
+    return qdiv(sum, T(4));
+}
+
+// (X - X^+)/2 (topologicalcharge.c:112-121)
+template <typename T>
+__device__ __forceinline__ Q<T> anti_hermitian(Q<T> X)
+{
+    Q<T> Xd = qdag(X);
+    X = Q<T>{X.c0 - Xd.c0, X.c1 - Xd.c1, X.c2 - Xd.c2, X.c3 - Xd.c3};
+    return qdiv(X, T(2));
+}
+
+template <typename T>
+__device__ __forceinline__ Q<T> improved_F(const Planes<T>& P, const Geo& g, const Site& x, int d1, int d2)
+{
+    const T k1 = T(1.5), k2 = T(-0.15), k3 = T(1.) / T(90.);
+    Q<T> U1 = qscale(clover_average(P, g, x, d1, d2, 1), k1);
+    Q<T> U2 = qscale(clover_average(P, g, x, d1, d2, 2), k2);
+    Q<T> U3 = qscale(clover_average(P, g, x, d1, d2, 3), k3);
+    Q<T> F = qadd(U1, U2);
+    F = qadd(F, U3);
+    return anti_hermitian(F);
+}
+
+template <typename T>
+__device__ __forceinline__ T improved_products(const Planes<T>& P, const Geo& g, const Site& x, int d1, int d2, int d3,
+                                               int d4)
+{
+    Q<T> F = improved_F(P, g, x, d1, d2);
+    Q<T> G = improved_F(P, g, x, d3, d4);
+    return T(2) * qmul(F, G).c0;
+}
+
+// 1x1 leaves (topologicalcharge.c:201-290)
+template <typename T>
+__device__ __forceinline__ Q<T> U_plpl(const Planes<T>& P, const Geo& g, const Site& x, int jj, int kk)
+{
+    Q<T> un0 = link(P, g, jj, x);
+    Q<T> un1 = link(P, g, kk, shift(g, x, jj, 1));
+    Q<T> un2 = qmul(un0, un1);
+    un0 = qdag(link(P, g, jj, shift(g, x, kk, 1)));
+    un1 = qmul(un2, un0);
+    un2 = qdag(link(P, g, kk, x));
+    return qmul(un1, un2);
+}
+
+template <typename T>
+__device__ __forceinline__ Q<T> U_mipl(const Planes<T>& P, const Geo& g, const Site& x, int jj, int kk)
+{
+    Site n = shift(g, x, jj, -1);
+    Q<T> un0 = qdag(link(P, g, jj, n));
+    Q<T> un1 = link(P, g, kk, n);
+    Q<T> un2 = qmul(un0, un1);
+    n = shift(g, n, kk, 1);
+    un1 = qmul(un2, link(P, g, jj, n));
+    un2 = qdag(link(P, g, kk, x));
+    return qmul(un1, un2);
+}
+
+template <typename T>
+__device__ __forceinline__ Q<T> U_mimi(const Planes<T>& P, const Geo& g, const Site& x, int jj, int kk)
+{
+    Site n = shift(g, x, jj, -1);
+    Q<T> un0 = qdag(link(P, g, jj, n));
+    n = shift(g, n, kk, -1);
+    Q<T> un2 = qmul_dag(un0, link(P, g, kk, n));
+    Q<T> un1 = qmul(un2, link(P, g, jj, n));
+    n = shift(g, x, kk, -1);
+    return qmul(un1, link(P, g, kk, n));
+}
+
+template <typename T>
+__device__ __forceinline__ Q<T> U_plmi(const Planes<T>& P, const Geo& g, const Site& x, int jj, int kk)
+{
+    Q<T> un0 = link(P, g, jj, x);
+    Site n = shift(g, shift(g, x, jj, 1), kk, -1);
+    Q<T> un2 = qmul_dag(un0, link(P, g, kk, n));
+    n = shift(g, x, kk, -1);
+    un0 = qdag(link(P, g, jj, n));
+    Q<T> un1 = qmul(un2, un0);
+    return qmul(un1, link(P, g, kk, n));
+}
+
+template <typename T>
+__device__ __forceinline__ Q<T> clover_1x1(const Planes<T>& P, const Geo& g, const Site& x, int d1, int d2)
+{
+    Q<T> p = U_plpl(P, g, x, d1, d2);
+    p = qadd(U_mipl(P, g, x, d2, d1), p);
+    p = qadd(U_mimi(P, g, x, d1, d2), p);
+    p = qadd(U_plmi(P, g, x, d2, d1), p);
+    return qdiv(p, T(4));
+}
+
+template <typename T>
+__device__ __forceinline__ T clover_products_1x1(const Planes<T>& P, const Geo& g, const Site& x, int d1, int d2,
+                                                 int d3, int d4)
+{
+    Q<T> C = clover_1x1(P, g, x, d1, d2);
+    Q<T> D = clover_1x1(P, g, x, d3, d4);
+    Q<T> Cd = qdag(C), Dd = qdag(D);
+    Q<T> P0{C.c0 - Cd.c0, C.c1 - Cd.c1, C.c2 - Cd.c2, C.c3 - Cd.c3};
+    Q<T> P1{D.c0 - Dd.c0, D.c1 - Dd.c1, D.c2 - Dd.c2, D.c3 - Dd.c3};
+    P0 = qdiv(P0, T(2));
+    P1 = qdiv(P1, T(2));
+    return T(2) * qmul(P0, P1).c0;
+}
+
+// KIND 0: improved (8 cp(0123) + 8 cp(0231) + 8 cp(0312)),
+// KIND 1: 1x1 clover (-cp(0132) - cp(1230) - cp(2031)).
+// partial[k] = sum q and partial[ncols + k] = sum |q| with k = y*gridDim.x +
+// tile, ncols = 2*gridDim.x; qx: density in lexicographic order (nullable).
+template <typename T, int KIND>
+__global__ void __launch_bounds__(BLOCK) k_qtop(Planes<T> P, Geo g, int remap, DD* __restrict__ partial,
+                                                double* __restrict__ qx)
+{
+    uint32_t tile = tile_id(remap);
+    uint32_t i = tile * BLOCK + threadIdx.x;
+    int p = blockIdx.y;
+    double q = 0.0;
+    if (i < g.vh) {
+        Site s = cb_site(g, i, p);
+        T v;
+        if (KIND == 0)
+            v = T(8.) * improved_products(P, g, s, 0, 1, 2, 3) + T(8.) * improved_products(P, g, s, 0, 2, 3, 1) +
+                T(8.) * improved_products(P, g, s, 0, 3, 1, 2);
+        else
+            v = -clover_products_1x1(P, g, s, 0, 1, 3, 2) - clover_products_1x1(P, g, s, 1, 2, 3, 0) -
+                clover_products_1x1(P, g, s, 2, 0, 3, 1);
+        q = (double)v;
+        if (qx) qx[lex_index(g, s)] = q;
+    }
+    DD a = block_sum_dd(DD{q, 0.0});
+    DD b = block_sum_dd(DD{fabs(q), 0.0});
+    if (threadIdx.x == 0) {
+        size_t k = (size_t)blockIdx.y * gridDim.x + tile;
+        partial[k] = a;
+        partial[2 * (size_t)gridDim.x + k] = b;
+    }
+}
+
+}  // namespace lgt

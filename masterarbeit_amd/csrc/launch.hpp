@@ -1,0 +1,54 @@
+// launch.hpp -- host-side launchers of the engine kernels, one explicit
+// instantiation per storage precision (lgt_f64.hip, lgt_f32.hip).  All
+// launches are asynchronous on the caller's stream; device pointers only.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "su2_device.hpp"
+
+namespace lgt {
+
+// raw plane pointers of one field: p[mu][parity]
+struct RawPlanes { void* p[4][2]; };
+
+struct WGroupHost { int R, Tmax, slot_off, pad; };
+
+// longest temporal extent of one Wilson-loop launch (LDS slots per wave)
+constexpr int WMAXT_HOST = 128;
+
+template <typename T> struct Launch {
+    static hipError_t cold(hipStream_t s, const Geo& g, const RawPlanes& P);
+    static hipError_t hot(hipStream_t s, const Geo& g, const RawPlanes& P, uint64_t seed);
+    static hipError_t from_aos(hipStream_t s, const Geo& g, const RawPlanes& P, const double* aos);
+    static hipError_t to_aos(hipStream_t s, const Geo& g, const RawPlanes& P, double* aos);
+    static hipError_t project(hipStream_t s, const Geo& g, const RawPlanes& P);
+    // acc_partial: nullable, receives ceil(vh/256) block sums of 1/count
+    static hipError_t heatbath(hipStream_t s, const Geo& g, const RawPlanes& P, int mu, int parity, double beta,
+                               uint64_t seed, uint32_t sweep, int remap, void* acc_partial);
+    static hipError_t overrelax(hipStream_t s, const Geo& g, const RawPlanes& P, int mu, int parity, int remap);
+    // partial: 2*ceil(vh/256) double-doubles
+    static hipError_t plaquette(hipStream_t s, const Geo& g, const RawPlanes& P, int remap, void* partial);
+    static hipError_t ape(hipStream_t s, const Geo& g, const RawPlanes& in, const RawPlanes& out, int mode, double sp,
+                          int remap);
+    // groups/slots: device tables; partial: nloop rows of 6*ceil(vh/256) columns
+    static hipError_t wloops(hipStream_t s, const Geo& g, const RawPlanes& P, const WGroupHost* groups, int ngroups,
+                             const int* slots, int remap, void* partial, int ncols);
+    // partial: 2 rows of 2*ceil(vh/256) columns (sum q, sum |q|)
+    static hipError_t qtop(hipStream_t s, const Geo& g, const RawPlanes& P, int kind, int remap, void* partial,
+                           double* density);
+};
+
+// precision-independent launchers (lgt_f64.hip)
+hipError_t launch_reduce_rows(hipStream_t s, const void* in, int nrows, int ncols, void* out);
+hipError_t launch_hb_harness(hipStream_t s, int n, const double* staples, double beta, const double* uniforms,
+                             const int* offsets, const int* lens, double* links, int* counts);
+hipError_t launch_or_harness(hipStream_t s, int n, const double* staples, const double* links_in, double* links);
+
+inline uint32_t grid_x(const Geo& g) { return (g.vh + 255u) / 256u; }
+
+extern template struct Launch<double>;
+extern template struct Launch<float>;
+
+}  // namespace lgt

@@ -1,0 +1,628 @@
+// lgt_api.hip -- C ABI of the engine (include/lgt_su2.h): handle, device
+// memory, stream, launch sequencing and host transfers.  No kernels here;
+// they are instantiated per precision in lgt_f64.hip / lgt_f32.hip.
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <map>
+#include <string>
+#include <vector>
+
+#include "../../include/lgt_su2.h"
+#include "launch.hpp"
+
+using namespace lgt;
+
+namespace {
+
+thread_local std::string g_last_error;
+
+int fail(int code, const char* fmt, ...)
+{
+    char buf[512];
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(buf, sizeof(buf), fmt, ap);
+    va_end(ap);
+    g_last_error = buf;
+    return code;
+}
+
+int hip_fail(hipError_t err, const char* what)
+{
+    return fail(LGT_EDEVICE, "%s: %s", what, hipGetErrorString(err));
+}
+
+#define HIP_TRY(call, what)                              \
+    do {                                                 \
+        hipError_t _e = (call);                          \
+        if (_e != hipSuccess) return hip_fail(_e, what); \
+    } while (0)
+
+struct ProfEvent {
+    int cls;
+    hipEvent_t a, b;
+};
+
+}  // namespace
+
+struct lgt_engine {
+    int dev = 0;
+    int L[4] = {0, 0, 0, 0};
+    int prec = LGT_FP64;
+    uint64_t seed = 0;
+    uint32_t sweep = 0;
+    int64_t vol = 0;
+    Geo geo;
+    size_t link_bytes = 0;
+    hipStream_t stream = nullptr;
+    void* field = nullptr;
+    void* field_alt = nullptr;
+    RawPlanes cur, alt;
+    void* partial = nullptr;
+    size_t partial_bytes = 0;
+    void* dres = nullptr;
+    size_t dres_bytes = 0;
+    double* hres = nullptr;
+    size_t hres_bytes = 0;
+    void* staging = nullptr;
+    size_t staging_bytes = 0;
+    int remap = 1;
+    int profile = 0;
+    std::vector<ProfEvent> events;
+    double prof_ms[LGT_K_COUNT] = {0};
+    int64_t prof_n[LGT_K_COUNT] = {0};
+    int64_t dev_bytes = 0;
+};
+
+namespace {
+
+int set_dev(lgt_engine* e)
+{
+    HIP_TRY(hipSetDevice(e->dev), "hipSetDevice");
+    return LGT_OK;
+}
+
+int ensure(lgt_engine* e, void** buf, size_t* have, size_t need, const char* what)
+{
+    if (*have >= need) return LGT_OK;
+    if (*buf) {
+        hipFree(*buf);
+        e->dev_bytes -= (int64_t)*have;
+        *buf = nullptr;
+        *have = 0;
+    }
+    hipError_t err = hipMalloc(buf, need);
+    if (err != hipSuccess) {
+        *buf = nullptr;
+        return fail(LGT_ENOMEM, "hipMalloc(%zu) for %s: %s", need, what, hipGetErrorString(err));
+    }
+    *have = need;
+    e->dev_bytes += (int64_t)need;
+    return LGT_OK;
+}
+
+int ensure_host(lgt_engine* e, size_t need)
+{
+    if (e->hres_bytes >= need) return LGT_OK;
+    if (e->hres) hipHostFree(e->hres);
+    e->hres = nullptr;
+    e->hres_bytes = 0;
+    hipError_t err = hipHostMalloc((void**)&e->hres, need, hipHostMallocDefault);
+    if (err != hipSuccess) return fail(LGT_ENOMEM, "hipHostMalloc(%zu): %s", need, hipGetErrorString(err));
+    e->hres_bytes = need;
+    return LGT_OK;
+}
+
+void set_planes(RawPlanes& P, void* base, uint32_t vh, size_t link_bytes)
+{
+    for (int mu = 0; mu < 4; mu++)
+        for (int p = 0; p < 2; p++) P.p[mu][p] = (char*)base + ((size_t)(mu * 2 + p) * vh) * link_bytes;
+}
+
+// runs one launcher, bracketed by events when profiling
+template <class F>
+int run(lgt_engine* e, int cls, const char* what, F&& f)
+{
+    hipError_t err;
+    if (e->profile) {
+        ProfEvent ev;
+        ev.cls = cls;
+        HIP_TRY(hipEventCreate(&ev.a), "hipEventCreate");
+        HIP_TRY(hipEventCreate(&ev.b), "hipEventCreate");
+        HIP_TRY(hipEventRecord(ev.a, e->stream), "hipEventRecord");
+        err = f();
+        HIP_TRY(hipEventRecord(ev.b, e->stream), "hipEventRecord");
+        e->events.push_back(ev);
+    } else {
+        err = f();
+    }
+    if (err != hipSuccess) return hip_fail(err, what);
+    return LGT_OK;
+}
+
+int flush_profile(lgt_engine* e)
+{
+    if (e->events.empty()) return LGT_OK;
+    HIP_TRY(hipStreamSynchronize(e->stream), "hipStreamSynchronize");
+    for (auto& ev : e->events) {
+        float ms = 0.f;
+        HIP_TRY(hipEventElapsedTime(&ms, ev.a, ev.b), "hipEventElapsedTime");
+        e->prof_ms[ev.cls] += ms;
+        e->prof_n[ev.cls] += 1;
+        hipEventDestroy(ev.a);
+        hipEventDestroy(ev.b);
+    }
+    e->events.clear();
+    return LGT_OK;
+}
+
+#define TRY(x)                     \
+    do {                           \
+        int _rc = (x);             \
+        if (_rc != LGT_OK) return _rc; \
+    } while (0)
+
+#define DISPATCH(e, call) ((e)->prec == LGT_FP64 ? Launch<double>::call : Launch<float>::call)
+
+uint32_t nbx(const lgt_engine* e) { return grid_x(e->geo); }
+
+// reduces `nrows` rows of `ncols` double-doubles to host doubles (hi + lo)
+int reduce_to_host(lgt_engine* e, int nrows, int ncols, double* out)
+{
+    TRY(ensure(e, &e->dres, &e->dres_bytes, sizeof(double) * 2 * (size_t)nrows, "results"));
+    TRY(ensure_host(e, sizeof(double) * 2 * (size_t)nrows));
+    TRY(run(e, LGT_K_REDUCE, "reduce", [&] { return launch_reduce_rows(e->stream, e->partial, nrows, ncols, e->dres); }));
+    HIP_TRY(hipMemcpyAsync(e->hres, e->dres, sizeof(double) * 2 * nrows, hipMemcpyDeviceToHost, e->stream),
+            "hipMemcpyAsync");
+    HIP_TRY(hipStreamSynchronize(e->stream), "hipStreamSynchronize");
+    for (int r = 0; r < nrows; r++) out[r] = e->hres[2 * r] + e->hres[2 * r + 1];
+    return LGT_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+const char* lgt_strerror(int status)
+{
+    switch (status) {
+    case LGT_OK: return "ok";
+    case LGT_EINVAL: return "invalid argument";
+    case LGT_ENOMEM: return "out of memory";
+    case LGT_EDEVICE: return "HIP device error";
+    default: return "unknown status";
+    }
+}
+
+const char* lgt_last_error(void) { return g_last_error.c_str(); }
+
+int lgt_create(const int extents[4], int precision, uint64_t seed, lgt_engine** out)
+{
+    if (!extents || !out) return fail(LGT_EINVAL, "lgt_create: null argument");
+    *out = nullptr;
+    if (precision != LGT_FP64 && precision != LGT_FP32) return fail(LGT_EINVAL, "lgt_create: precision %d", precision);
+    int64_t vol = 1;
+    for (int d = 0; d < 4; d++) {
+        if (extents[d] < 2 || (extents[d] & 1) || extents[d] > 4096)
+            return fail(LGT_EINVAL, "lgt_create: extent[%d] = %d must be even and in [2, 4096]", d, extents[d]);
+        vol *= extents[d];
+    }
+    if (extents[3] < 4) return fail(LGT_EINVAL, "lgt_create: extent[3] = %d must be >= 4", extents[3]);
+    if (vol / 2 >= (int64_t(1) << 31)) return fail(LGT_EINVAL, "lgt_create: volume %lld too large", (long long)vol);
+
+    lgt_engine* e = new lgt_engine();
+    hipError_t err = hipGetDevice(&e->dev);
+    if (err != hipSuccess) {
+        delete e;
+        return hip_fail(err, "hipGetDevice");
+    }
+    for (int d = 0; d < 4; d++) e->L[d] = extents[d];
+    e->prec = precision;
+    e->seed = seed;
+    e->vol = vol;
+    Geo& g = e->geo;
+    for (int d = 0; d < 4; d++) g.L[d] = extents[d];
+    g.hz = extents[3] / 2;
+    g.vh = (uint32_t)(vol / 2);
+    g.stride[3] = 0;
+    g.stride[2] = (uint32_t)g.hz;
+    g.stride[1] = (uint32_t)extents[2] * g.hz;
+    g.stride[0] = (uint32_t)extents[1] * extents[2] * g.hz;
+    g.dz = make_fastdiv((uint32_t)g.hz);
+    g.dy = make_fastdiv((uint32_t)extents[2]);
+    g.dx = make_fastdiv((uint32_t)extents[1]);
+    e->link_bytes = (precision == LGT_FP64) ? 32 : 16;
+
+    err = hipStreamCreateWithFlags(&e->stream, hipStreamNonBlocking);
+    if (err != hipSuccess) {
+        delete e;
+        return hip_fail(err, "hipStreamCreate");
+    }
+    size_t bytes = (size_t)4 * vol * e->link_bytes;
+    err = hipMalloc(&e->field, bytes);
+    if (err != hipSuccess) {
+        hipStreamDestroy(e->stream);
+        delete e;
+        return fail(LGT_ENOMEM, "hipMalloc(%zu) for the link field: %s", bytes, hipGetErrorString(err));
+    }
+    e->dev_bytes = (int64_t)bytes;
+    set_planes(e->cur, e->field, g.vh, e->link_bytes);
+    int rc = lgt_init_field(e, 0);
+    if (rc != LGT_OK) {
+        lgt_destroy(e);
+        return rc;
+    }
+    *out = e;
+    return LGT_OK;
+}
+
+int lgt_destroy(lgt_engine* e)
+{
+    if (!e) return LGT_OK;
+    hipSetDevice(e->dev);
+    if (e->stream) hipStreamSynchronize(e->stream);
+    for (auto& ev : e->events) {
+        hipEventDestroy(ev.a);
+        hipEventDestroy(ev.b);
+    }
+    if (e->field) hipFree(e->field);
+    if (e->field_alt) hipFree(e->field_alt);
+    if (e->partial) hipFree(e->partial);
+    if (e->dres) hipFree(e->dres);
+    if (e->staging) hipFree(e->staging);
+    if (e->hres) hipHostFree(e->hres);
+    if (e->stream) hipStreamDestroy(e->stream);
+    delete e;
+    return LGT_OK;
+}
+
+int lgt_set_option(lgt_engine* e, int option, int value)
+{
+    if (!e) return fail(LGT_EINVAL, "lgt_set_option: null engine");
+    switch (option) {
+    case LGT_OPT_XCD_REMAP: e->remap = value ? 1 : 0; return LGT_OK;
+    case LGT_OPT_PROFILE:
+        if (!value) TRY(flush_profile(e));
+        e->profile = value ? 1 : 0;
+        return LGT_OK;
+    default: return fail(LGT_EINVAL, "lgt_set_option: unknown option %d", option);
+    }
+}
+
+int lgt_get_extents(const lgt_engine* e, int extents[4], int* precision)
+{
+    if (!e) return fail(LGT_EINVAL, "lgt_get_extents: null engine");
+    if (extents)
+        for (int d = 0; d < 4; d++) extents[d] = e->L[d];
+    if (precision) *precision = e->prec;
+    return LGT_OK;
+}
+
+int64_t lgt_device_bytes(const lgt_engine* e) { return e ? e->dev_bytes : 0; }
+
+int lgt_init_field(lgt_engine* e, int start)
+{
+    if (!e || (start != 0 && start != 1)) return fail(LGT_EINVAL, "lgt_init_field: start must be 0 or 1");
+    TRY(set_dev(e));
+    if (start == 0)
+        return run(e, LGT_K_FIELD, "cold start", [&] { return DISPATCH(e, cold(e->stream, e->geo, e->cur)); });
+    return run(e, LGT_K_FIELD, "hot start", [&] { return DISPATCH(e, hot(e->stream, e->geo, e->cur, e->seed)); });
+}
+
+int lgt_upload(lgt_engine* e, const double* aos)
+{
+    if (!e || !aos) return fail(LGT_EINVAL, "lgt_upload: null argument");
+    TRY(set_dev(e));
+    size_t bytes = sizeof(double) * 16 * (size_t)e->vol;
+    TRY(ensure(e, &e->staging, &e->staging_bytes, bytes, "staging"));
+    HIP_TRY(hipMemcpyAsync(e->staging, aos, bytes, hipMemcpyHostToDevice, e->stream), "hipMemcpyAsync H2D");
+    TRY(run(e, LGT_K_FIELD, "from_aos",
+            [&] { return DISPATCH(e, from_aos(e->stream, e->geo, e->cur, (const double*)e->staging)); }));
+    HIP_TRY(hipStreamSynchronize(e->stream), "hipStreamSynchronize");
+    return LGT_OK;
+}
+
+int lgt_download(lgt_engine* e, double* aos)
+{
+    if (!e || !aos) return fail(LGT_EINVAL, "lgt_download: null argument");
+    TRY(set_dev(e));
+    size_t bytes = sizeof(double) * 16 * (size_t)e->vol;
+    TRY(ensure(e, &e->staging, &e->staging_bytes, bytes, "staging"));
+    TRY(run(e, LGT_K_FIELD, "to_aos", [&] { return DISPATCH(e, to_aos(e->stream, e->geo, e->cur, (double*)e->staging)); }));
+    HIP_TRY(hipMemcpyAsync(aos, e->staging, bytes, hipMemcpyDeviceToHost, e->stream), "hipMemcpyAsync D2H");
+    HIP_TRY(hipStreamSynchronize(e->stream), "hipStreamSynchronize");
+    return LGT_OK;
+}
+
+int lgt_heatbath_subsweep(lgt_engine* e, int mu, int parity, double beta, double* acc_sum)
+{
+    if (!e || mu < 0 || mu > 3 || parity < 0 || parity > 1) return fail(LGT_EINVAL, "lgt_heatbath_subsweep: mu/parity");
+    TRY(set_dev(e));
+    void* ap = nullptr;
+    if (acc_sum) {
+        TRY(ensure(e, &e->partial, &e->partial_bytes, sizeof(double) * 2 * nbx(e), "partials"));
+        ap = e->partial;
+    }
+    TRY(run(e, LGT_K_HEATBATH, "heatbath", [&] {
+        return DISPATCH(e, heatbath(e->stream, e->geo, e->cur, mu, parity, beta, e->seed, e->sweep, e->remap, ap));
+    }));
+    if (acc_sum) TRY(reduce_to_host(e, 1, (int)nbx(e), acc_sum));
+    return LGT_OK;
+}
+
+int lgt_overrelax_subsweep(lgt_engine* e, int mu, int parity)
+{
+    if (!e || mu < 0 || mu > 3 || parity < 0 || parity > 1) return fail(LGT_EINVAL, "lgt_overrelax_subsweep: mu/parity");
+    TRY(set_dev(e));
+    return run(e, LGT_K_OVERRELAX, "overrelax",
+               [&] { return DISPATCH(e, overrelax(e->stream, e->geo, e->cur, mu, parity, e->remap)); });
+}
+
+int lgt_project(lgt_engine* e)
+{
+    if (!e) return fail(LGT_EINVAL, "lgt_project: null engine");
+    TRY(set_dev(e));
+    return run(e, LGT_K_PROJECT, "project", [&] { return DISPATCH(e, project(e->stream, e->geo, e->cur)); });
+}
+
+int lgt_set_sweep_index(lgt_engine* e, uint32_t sweep)
+{
+    if (!e) return fail(LGT_EINVAL, "lgt_set_sweep_index: null engine");
+    e->sweep = sweep;
+    return LGT_OK;
+}
+
+uint32_t lgt_get_sweep_index(const lgt_engine* e) { return e ? e->sweep : 0u; }
+
+int lgt_update(lgt_engine* e, int nsweeps, int n_or, double beta, double* acceptance)
+{
+    if (!e || nsweeps < 0 || n_or < 0) return fail(LGT_EINVAL, "lgt_update: nsweeps/n_or must be >= 0");
+    if (!(beta >= 0.0) || !std::isfinite(beta)) return fail(LGT_EINVAL, "lgt_update: beta = %g", beta);
+    TRY(set_dev(e));
+    const uint32_t nb = nbx(e);
+    std::vector<double> acc_rows;
+    if (acceptance) {
+        TRY(ensure(e, &e->partial, &e->partial_bytes, sizeof(double) * 2 * 8 * (size_t)nb, "partials"));
+        acc_rows.resize(nsweeps);
+    }
+    for (int s = 0; s < nsweeps; s++) {
+        for (int mu = 0; mu < 4; mu++)
+            for (int p = 0; p < 2; p++) {
+                void* ap = acceptance ? (char*)e->partial + sizeof(double) * 2 * (size_t)nb * (mu * 2 + p) : nullptr;
+                TRY(run(e, LGT_K_HEATBATH, "heatbath", [&] {
+                    return DISPATCH(e, heatbath(e->stream, e->geo, e->cur, mu, p, beta, e->seed, e->sweep, e->remap, ap));
+                }));
+            }
+        if (acceptance) TRY(reduce_to_host(e, 1, (int)(8 * nb), &acc_rows[s]));
+        for (int k = 0; k < n_or; k++)
+            for (int mu = 0; mu < 4; mu++)
+                for (int p = 0; p < 2; p++)
+                    TRY(run(e, LGT_K_OVERRELAX, "overrelax",
+                            [&] { return DISPATCH(e, overrelax(e->stream, e->geo, e->cur, mu, p, e->remap)); }));
+        if (n_or == 0) TRY(run(e, LGT_K_PROJECT, "project", [&] { return DISPATCH(e, project(e->stream, e->geo, e->cur)); }));
+        e->sweep++;
+    }
+    if (acceptance) {
+        double a = 0.0;
+        for (double v : acc_rows) a += v / (double)(4 * e->vol);
+        *acceptance = nsweeps > 0 ? a / nsweeps : 0.0;
+    }
+    HIP_TRY(hipStreamSynchronize(e->stream), "hipStreamSynchronize");
+    return LGT_OK;
+}
+
+int lgt_plaquette(lgt_engine* e, double* plaq)
+{
+    if (!e || !plaq) return fail(LGT_EINVAL, "lgt_plaquette: null argument");
+    TRY(set_dev(e));
+    const uint32_t nb = nbx(e);
+    TRY(ensure(e, &e->partial, &e->partial_bytes, sizeof(double) * 2 * 2 * (size_t)nb, "partials"));
+    TRY(run(e, LGT_K_PLAQUETTE, "plaquette",
+            [&] { return DISPATCH(e, plaquette(e->stream, e->geo, e->cur, e->remap, e->partial)); }));
+    double s = 0.0;
+    TRY(reduce_to_host(e, 1, (int)(2 * nb), &s));
+    *plaq = s / (double)(2 * 6 * e->vol);
+    return LGT_OK;
+}
+
+int lgt_gauge_action(lgt_engine* e, double beta, double* action)
+{
+    if (!action) return fail(LGT_EINVAL, "lgt_gauge_action: null argument");
+    double p = 0.0;
+    TRY(lgt_plaquette(e, &p));
+    *action = beta * 6 * (double)e->vol * (1. - p);
+    return LGT_OK;
+}
+
+int lgt_ape(lgt_engine* e, int mode, int nsteps, double sp)
+{
+    if (!e || mode < 0 || mode > 2 || nsteps < 0) return fail(LGT_EINVAL, "lgt_ape: mode %d nsteps %d", mode, nsteps);
+    TRY(set_dev(e));
+    if (nsteps == 0) return LGT_OK;
+    if (!e->field_alt) {
+        size_t bytes = (size_t)4 * e->vol * e->link_bytes;
+        size_t have = 0;
+        TRY(ensure(e, &e->field_alt, &have, bytes, "APE buffer"));
+        set_planes(e->alt, e->field_alt, e->geo.vh, e->link_bytes);
+    }
+    int mu_lo = (mode == LGT_APE_SPATIAL) ? 1 : 0, mu_hi = (mode == LGT_APE_TEMPORAL) ? 0 : 3;
+    for (int s = 0; s < nsteps; s++) {
+        TRY(run(e, LGT_K_APE, "ape", [&] { return DISPATCH(e, ape(e->stream, e->geo, e->cur, e->alt, mode, sp, e->remap)); }));
+        for (int mu = mu_lo; mu <= mu_hi; mu++)
+            for (int p = 0; p < 2; p++) std::swap(e->cur.p[mu][p], e->alt.p[mu][p]);
+    }
+    HIP_TRY(hipStreamSynchronize(e->stream), "hipStreamSynchronize");
+    return LGT_OK;
+}
+
+int lgt_wilson_loops(lgt_engine* e, int nloop, const int* R, const int* T, double* w)
+{
+    if (!e || nloop <= 0 || !R || !T || !w) return fail(LGT_EINVAL, "lgt_wilson_loops: empty loop list");
+    for (int i = 0; i < nloop; i++) {
+        if (T[i] < 1 || T[i] > e->L[0] - 1 || T[i] > WMAXT_HOST)
+            return fail(LGT_EINVAL, "lgt_wilson_loops: T = %d outside [1, %d] (lattice too small)", T[i], e->L[0] - 1);
+        if (R[i] < 1) return fail(LGT_EINVAL, "lgt_wilson_loops: R = %d < 1", R[i]);
+    }
+    TRY(set_dev(e));
+    // unique (R,T) pairs, grouped by R in order of first appearance
+    std::map<std::pair<int, int>, int> uniq;
+    std::vector<int> order_R;
+    std::map<int, int> tmax;
+    for (int i = 0; i < nloop; i++) {
+        auto key = std::make_pair(R[i], T[i]);
+        if (!uniq.count(key)) {
+            int id = (int)uniq.size();
+            uniq[key] = id;
+        }
+        if (!tmax.count(R[i])) {
+            order_R.push_back(R[i]);
+            tmax[R[i]] = 0;
+        }
+        tmax[R[i]] = std::max(tmax[R[i]], T[i]);
+    }
+    std::vector<WGroupHost> groups;
+    std::vector<int> slots;
+    for (int r : order_R) {
+        WGroupHost gh{r, tmax[r], (int)slots.size(), 0};
+        for (int t = 1; t <= tmax[r]; t++) {
+            auto it = uniq.find(std::make_pair(r, t));
+            slots.push_back(it == uniq.end() ? -1 : it->second);
+        }
+        groups.push_back(gh);
+    }
+    const int nuniq = (int)uniq.size();
+    const int ncols = 6 * (int)nbx(e);
+    size_t tab_bytes = sizeof(WGroupHost) * groups.size() + sizeof(int) * slots.size();
+    size_t part_bytes = sizeof(double) * 2 * (size_t)nuniq * ncols;
+    TRY(ensure(e, &e->partial, &e->partial_bytes, part_bytes + tab_bytes + 256, "partials"));
+    char* tab = (char*)e->partial + ((part_bytes + 255) & ~(size_t)255);
+    WGroupHost* dgroups = (WGroupHost*)tab;
+    int* dslots = (int*)(tab + sizeof(WGroupHost) * groups.size());
+    HIP_TRY(hipMemcpyAsync(dgroups, groups.data(), sizeof(WGroupHost) * groups.size(), hipMemcpyHostToDevice, e->stream),
+            "hipMemcpyAsync");
+    HIP_TRY(hipMemcpyAsync(dslots, slots.data(), sizeof(int) * slots.size(), hipMemcpyHostToDevice, e->stream),
+            "hipMemcpyAsync");
+    TRY(run(e, LGT_K_WLOOP, "wilson loops", [&] {
+        return DISPATCH(e, wloops(e->stream, e->geo, e->cur, dgroups, (int)groups.size(), dslots, e->remap, e->partial, ncols));
+    }));
+    std::vector<double> sums(nuniq);
+    TRY(reduce_to_host(e, nuniq, ncols, sums.data()));
+    for (int i = 0; i < nloop; i++) w[i] = sums[uniq[std::make_pair(R[i], T[i])]] / (double)(3 * e->vol * 2);
+    return LGT_OK;
+}
+
+int lgt_qtop(lgt_engine* e, int kind, double* q, double* abs_sum, double* density)
+{
+    if (!e || !q || (kind != LGT_Q_IMPROVED && kind != LGT_Q_CLOVER)) return fail(LGT_EINVAL, "lgt_qtop: kind %d", kind);
+    TRY(set_dev(e));
+    const uint32_t nb = nbx(e);
+    TRY(ensure(e, &e->partial, &e->partial_bytes, sizeof(double) * 2 * 4 * (size_t)nb, "partials"));
+    double* dq = nullptr;
+    if (density) {
+        TRY(ensure(e, &e->staging, &e->staging_bytes, sizeof(double) * (size_t)e->vol, "staging"));
+        dq = (double*)e->staging;
+    }
+    TRY(run(e, LGT_K_QTOP, "qtop",
+            [&] { return DISPATCH(e, qtop(e->stream, e->geo, e->cur, kind, e->remap, e->partial, dq)); }));
+    double s[2];
+    TRY(reduce_to_host(e, 2, (int)(2 * nb), s));
+    const double PI = 3.14159265358979323846;
+    double norm = (kind == LGT_Q_IMPROVED) ? (32. * PI * PI) : (4. * PI * PI);
+    *q = s[0] / norm;
+    if (abs_sum) *abs_sum = s[1] / norm;
+    if (density) {
+        HIP_TRY(hipMemcpy(density, dq, sizeof(double) * (size_t)e->vol, hipMemcpyDeviceToHost), "hipMemcpy D2H");
+    }
+    return LGT_OK;
+}
+
+int lgt_heatbath_harness(int n, const double* staples, double beta, const double* uniforms, const int* offsets,
+                         const int* lens, double* links, int* counts)
+{
+    if (n <= 0 || !staples || !uniforms || !offsets || !lens || !links || !counts)
+        return fail(LGT_EINVAL, "lgt_heatbath_harness: bad arguments");
+    int64_t nuni = 0;
+    for (int i = 0; i < n; i++) {
+        if (offsets[i] < 0 || lens[i] < 0) return fail(LGT_EINVAL, "lgt_heatbath_harness: negative offset");
+        nuni = std::max<int64_t>(nuni, (int64_t)offsets[i] + lens[i]);
+    }
+    double *ds = nullptr, *du = nullptr, *dl = nullptr;
+    int *doff = nullptr, *dlen = nullptr, *dc = nullptr;
+    int rc = LGT_OK;
+    hipError_t err = hipSuccess;
+    if ((err = hipMalloc(&ds, sizeof(double) * 4 * n)) != hipSuccess || (err = hipMalloc(&du, sizeof(double) * (nuni + 1))) != hipSuccess ||
+        (err = hipMalloc(&dl, sizeof(double) * 4 * n)) != hipSuccess || (err = hipMalloc(&doff, sizeof(int) * n)) != hipSuccess ||
+        (err = hipMalloc(&dlen, sizeof(int) * n)) != hipSuccess || (err = hipMalloc(&dc, sizeof(int) * n)) != hipSuccess) {
+        rc = fail(LGT_ENOMEM, "lgt_heatbath_harness: hipMalloc: %s", hipGetErrorString(err));
+    }
+    if (rc == LGT_OK) {
+        hipMemcpy(ds, staples, sizeof(double) * 4 * n, hipMemcpyHostToDevice);
+        hipMemcpy(du, uniforms, sizeof(double) * nuni, hipMemcpyHostToDevice);
+        hipMemcpy(doff, offsets, sizeof(int) * n, hipMemcpyHostToDevice);
+        hipMemcpy(dlen, lens, sizeof(int) * n, hipMemcpyHostToDevice);
+        err = launch_hb_harness(nullptr, n, ds, beta, du, doff, dlen, dl, dc);
+        if (err == hipSuccess) err = hipDeviceSynchronize();
+        if (err == hipSuccess) err = hipMemcpy(links, dl, sizeof(double) * 4 * n, hipMemcpyDeviceToHost);
+        if (err == hipSuccess) err = hipMemcpy(counts, dc, sizeof(int) * n, hipMemcpyDeviceToHost);
+        if (err != hipSuccess) rc = hip_fail(err, "lgt_heatbath_harness");
+    }
+    hipFree(ds); hipFree(du); hipFree(dl); hipFree(doff); hipFree(dlen); hipFree(dc);
+    return rc;
+}
+
+int lgt_overrelax_harness(int n, const double* staples, const double* links_in, double* links_out)
+{
+    if (n <= 0 || !staples || !links_in || !links_out) return fail(LGT_EINVAL, "lgt_overrelax_harness: bad arguments");
+    double *ds = nullptr, *di = nullptr, *dl = nullptr;
+    int rc = LGT_OK;
+    hipError_t err = hipSuccess;
+    if ((err = hipMalloc(&ds, sizeof(double) * 4 * n)) != hipSuccess || (err = hipMalloc(&di, sizeof(double) * 4 * n)) != hipSuccess ||
+        (err = hipMalloc(&dl, sizeof(double) * 4 * n)) != hipSuccess)
+        rc = fail(LGT_ENOMEM, "lgt_overrelax_harness: hipMalloc: %s", hipGetErrorString(err));
+    if (rc == LGT_OK) {
+        hipMemcpy(ds, staples, sizeof(double) * 4 * n, hipMemcpyHostToDevice);
+        hipMemcpy(di, links_in, sizeof(double) * 4 * n, hipMemcpyHostToDevice);
+        err = launch_or_harness(nullptr, n, ds, di, dl);
+        if (err == hipSuccess) err = hipDeviceSynchronize();
+        if (err == hipSuccess) err = hipMemcpy(links_out, dl, sizeof(double) * 4 * n, hipMemcpyDeviceToHost);
+        if (err != hipSuccess) rc = hip_fail(err, "lgt_overrelax_harness");
+    }
+    hipFree(ds); hipFree(di); hipFree(dl);
+    return rc;
+}
+
+int lgt_profile_read(lgt_engine* e, int kernel_class, double* ms_total, int64_t* launches)
+{
+    if (!e || kernel_class < 0 || kernel_class >= LGT_K_COUNT) return fail(LGT_EINVAL, "lgt_profile_read: class");
+    TRY(set_dev(e));
+    TRY(flush_profile(e));
+    if (ms_total) *ms_total = e->prof_ms[kernel_class];
+    if (launches) *launches = e->prof_n[kernel_class];
+    return LGT_OK;
+}
+
+int lgt_profile_reset(lgt_engine* e)
+{
+    if (!e) return fail(LGT_EINVAL, "lgt_profile_reset: null engine");
+    TRY(set_dev(e));
+    TRY(flush_profile(e));
+    for (int k = 0; k < LGT_K_COUNT; k++) {
+        e->prof_ms[k] = 0.0;
+        e->prof_n[k] = 0;
+    }
+    return LGT_OK;
+}
+
+int lgt_sync(lgt_engine* e)
+{
+    if (!e) return fail(LGT_EINVAL, "lgt_sync: null engine");
+    TRY(set_dev(e));
+    HIP_TRY(hipStreamSynchronize(e->stream), "hipStreamSynchronize");
+    return LGT_OK;
+}
+
+}  // extern "C"

@@ -61,6 +61,18 @@ void ref_get_field(double *aos)
         }
 }
 
+void ref_get_link(long n, int mu, double *a)
+{
+    a[0] = pu[n][mu]->c0; a[1] = pu[n][mu]->c1; a[2] = pu[n][mu]->c2; a[3] = pu[n][mu]->c3;
+}
+
+void ref_su2_random(double *a)
+{
+    su2mat u;
+    su2RandomMatrix(&u);
+    a[0] = u.c0; a[1] = u.c1; a[2] = u.c2; a[3] = u.c3;
+}
+
 void ref_init_field(int flag) { releaseGaugeField(); initGaugeField(flag); }
 
 /* RNG access: state save/restore lets a caller replay exactly the uniforms a

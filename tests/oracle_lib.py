@@ -213,6 +213,8 @@ class RefLib:
                 "ref_set_field": (None, [_dp]),
                 "ref_get_field": (None, [_dp]),
                 "ref_init_field": (None, [C.c_int]),
+                "ref_get_link": (None, [C.c_long, C.c_int, _dp]),
+                "ref_su2_random": (None, [_dp]),
                 "ref_rng_state_size": (C.c_int, []),
                 "ref_rng_get": (None, [_ip]),
                 "ref_rng_reset": (None, [_ip]),
@@ -267,6 +269,16 @@ class RefLib:
         A = np.zeros(4)
         self.lib.ref_staple(n, mu, _ptr(A))
         return A
+
+    def get_link(self, n, mu):
+        a = np.zeros(4)
+        self.lib.ref_get_link(n, mu, _ptr(a))
+        return a
+
+    def su2_random(self):
+        a = np.zeros(4)
+        self.lib.ref_su2_random(_ptr(a))
+        return a
 
     def wilson(self, R, T):
         R = np.ascontiguousarray(R, dtype=np.int32)

@@ -1,0 +1,335 @@
+"""Parity of the HIP engine with the FP64 oracle and the reference fixtures,
+through the C ABI (masterarbeit_amd/liblgt_su2.so).
+
+Tolerances: FP64 per-link and per-site results are expected bit-identical to
+the oracle (same operation order, no FMA contraction, IEEE division and
+square root); global sums are double-double on the GPU and compensated on the
+CPU, compared at 1e-12 relative (|dQ| <= 1e-13 sum|q| for the charge).  The
+heatbath's transcendentals (log, cos, acos, sin) come from the device math
+library, so heatbath links agree with glibc's to a few ulp with identical
+trial counts.  FP32-storage results are compared with FP64 at the stated
+FP32 tolerances."""
+import os
+
+import numpy as np
+import pytest
+
+import fields as F
+from oracle_lib import Oracle
+
+pytestmark = pytest.mark.gpu
+
+GOLDEN = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+LATS = [(4, 4, 4, 4), (6, 4, 4, 8), (8, 8, 8, 8)]
+
+
+@pytest.fixture(scope="module")
+def lgt():
+    import masterarbeit_amd as m
+    m.load_library()
+    return m
+
+
+def load(name):
+    return dict(np.load(os.path.join(GOLDEN, name), allow_pickle=False))
+
+
+def engine(lgt, L, prec=0, seed=1, U=None):
+    e = lgt.Engine(L, prec, seed)
+    if U is not None:
+        e.upload(U)
+    return e
+
+
+def thermal_field(L, seed=3, nsweeps=2):
+    o = Oracle(L)
+    U = o.hot(seed)
+    for s in range(nsweeps):
+        o.engine_sweep(U, 3.09, 1, seed, s)
+    return o, U
+
+
+def ulp_diff(a, b):
+    a = np.asarray(a, dtype=np.float64)
+    b = np.asarray(b, dtype=np.float64)
+    return np.abs(a - b) / np.spacing(np.maximum(np.abs(a), np.abs(b)))
+
+
+# ---------------------------------------------------------------------------
+# field life cycle
+# ---------------------------------------------------------------------------
+@pytest.mark.parametrize("L", LATS)
+def test_upload_download_roundtrip(lgt, L):
+    o = Oracle(L)
+    U = o.hot(4)
+    with engine(lgt, L, 0, U=U) as e:
+        np.testing.assert_array_equal(e.download(), U)
+    with engine(lgt, L, 1, U=U) as e:
+        np.testing.assert_array_equal(e.download(), U.astype(np.float32).astype(np.float64))
+
+
+@pytest.mark.parametrize("prec", [0, 1])
+def test_cold_start(lgt, prec):
+    L = (6, 4, 4, 8)
+    with engine(lgt, L, prec) as e:
+        U = e.download()
+        np.testing.assert_array_equal(U, Oracle(L).cold())
+        assert e.plaquette() == 1.0
+        assert e.qtop()[0] == 0.0
+
+
+@pytest.mark.parametrize("L", LATS)
+def test_hot_start_matches_oracle(lgt, L):
+    with engine(lgt, L, 0, seed=12345) as e:
+        e.init_field(1)
+        G = e.download()
+    C = Oracle(L).hot(12345)
+    assert np.max(np.abs(G - C)) <= 4e-16
+
+
+# ---------------------------------------------------------------------------
+# observables
+# ---------------------------------------------------------------------------
+@pytest.mark.parametrize("L", LATS)
+def test_plaquette_parity(lgt, L):
+    o, U = thermal_field(L)
+    with engine(lgt, L, 0, U=U) as e:
+        p = e.plaquette()
+    assert p == pytest.approx(o.plaquette(U, acc=True), rel=1e-14)
+    assert p == pytest.approx(o.plaquette(U), rel=1e-12)
+
+
+def test_plaquette_reference_fixture(lgt):
+    for name in ["ref_l4.npz", "ref_a6448.npz"]:
+        fx = load(name)
+        L = tuple(int(x) for x in fx["L"])
+        with engine(lgt, L, 0, U=fx["U0"]) as e:
+            assert e.plaquette() == pytest.approx(float(fx["plaq"]), rel=1e-13)
+
+
+@pytest.mark.parametrize("L", LATS)
+@pytest.mark.parametrize("mode,sp", [(0, 0.25), (1, 1.0 / 6.0), (2, 0.3)])
+def test_ape_parity(lgt, L, mode, sp):
+    o, U = thermal_field(L)
+    for nsteps in (1, 7):
+        V = U.copy()
+        o.ape(V, mode, nsteps, sp)
+        with engine(lgt, L, 0, U=U) as e:
+            e.ape(mode, nsteps, sp)
+            G = e.download()
+        d = np.max(np.abs(G - V))
+        print(f"APE L={L} mode={mode} n={nsteps}: max|dU| = {d:.3g} exact={np.array_equal(G, V)}")
+        assert d <= 1e-14
+
+
+def test_ape_reference_fixture(lgt):
+    fx = load("ref_l4.npz")
+    L = tuple(int(x) for x in fx["L"])
+    for tag in ["ape_spatial", "ape_all", "ape_temporal"]:
+        mode, nst, sp = fx[tag + "_args"]
+        with engine(lgt, L, 0, U=fx["U0"]) as e:
+            e.ape(int(mode), int(nst), float(sp))
+            G = e.download()
+            assert np.max(np.abs(G - fx[tag])) <= 1e-14
+            w = e.wilson_loops(fx[tag + "_wR"], fx[tag + "_wT"])
+            np.testing.assert_allclose(w, fx[tag + "_w"], rtol=1e-12, atol=1e-14)
+
+
+@pytest.mark.parametrize("L", LATS)
+def test_wilson_loop_parity(lgt, L):
+    o, U = thermal_field(L)
+    o.ape(U, 0, 4, 0.25)     # smeared: loops well above rounding noise
+    tmax = min(L[0] - 1, 6)
+    R, T = np.meshgrid(np.arange(1, 5), np.arange(1, tmax + 1), indexing="ij")
+    R, T = R.ravel(), T.ravel()
+    with engine(lgt, L, 0, U=U) as e:
+        w = e.wilson_loops(R, T)
+        w_perm = e.wilson_loops(R[::-1], T[::-1])       # order of the list does not matter
+    ref = o.wilson(U, R, T)
+    np.testing.assert_allclose(w, ref, rtol=1e-12, atol=1e-15)
+    np.testing.assert_array_equal(w_perm[::-1], w)
+
+
+def test_wilson_loop_reference_fixture(lgt):
+    for name in ["ref_l4.npz", "ref_a6448.npz"]:
+        fx = load(name)
+        L = tuple(int(x) for x in fx["L"])
+        with engine(lgt, L, 0, U=fx["U0"]) as e:
+            np.testing.assert_allclose(e.wilson_loops(fx["wR"], fx["wT"]), fx["w"], rtol=1e-12, atol=1e-15)
+
+
+def test_wilson_loop_rejects_long_t(lgt):
+    with engine(lgt, (4, 4, 4, 4), 0) as e:
+        with pytest.raises(lgt.LgtError):
+            e.wilson_loops([1], [4])                       # wilson.c:290-291: T <= L0 - 1
+
+
+@pytest.mark.parametrize("L", LATS)
+@pytest.mark.parametrize("kind", [0, 1])
+def test_qtop_parity(lgt, L, kind):
+    o, U = thermal_field(L)
+    o.ape(U, 1, 3, 1.0 / 6.0)
+    q_ref, s_ref, qx_ref = o.qtop(U, "improved" if kind == 0 else "clover", density=True)
+    with engine(lgt, L, 0, U=U) as e:
+        q, s, qx = e.qtop(kind, density=True)
+    d = np.max(np.abs(qx - qx_ref))
+    print(f"Q L={L} kind={kind}: Q={q:.6g} ref={q_ref:.6g} max|dq(x)|={d:.3g} exact={np.array_equal(qx, qx_ref)}")
+    assert d <= 1e-15 * max(1.0, np.max(np.abs(qx_ref)))
+    assert abs(q - q_ref) <= 1e-13 * s_ref
+    assert s == pytest.approx(s_ref, rel=1e-12)
+
+
+def test_qtop_reference_fixture(lgt):
+    for name in ["ref_l4.npz", "ref_a6448.npz"]:
+        fx = load(name)
+        L = tuple(int(x) for x in fx["L"])
+        with engine(lgt, L, 0, U=fx["U0"]) as e:
+            q, s = e.qtop(0)
+            assert abs(q - float(fx["qimp"])) <= 1e-13 * s
+            q, s = e.qtop(1)
+            assert abs(q - float(fx["qclov"])) <= 1e-13 * s
+
+
+def test_qtop_abelian_flux_closed_form(lgt):
+    L = (6, 4, 4, 8)
+    th1, phi = F.flux_angles(L, 0, 1, 1)
+    th2, psi = F.flux_angles(L, 2, 3, 2)
+    U = F.abelian_field(th1 + th2)
+    vol = int(np.prod(L))
+    with engine(lgt, L, 0, U=U) as e:
+        q, _ = e.qtop(0)
+        qc, _ = e.qtop(1)
+    assert q == pytest.approx(vol * 8 * (-2 * F.improved_f(phi) * F.improved_f(psi)) / (32 * np.pi ** 2), rel=1e-12)
+    assert qc == pytest.approx(vol * (-2 * np.sin(phi) * np.sin(psi)) / (4 * np.pi ** 2), rel=1e-12)
+
+
+# ---------------------------------------------------------------------------
+# update: per-link harnesses, sub-sweep replays, full sweeps
+# ---------------------------------------------------------------------------
+def test_heatbath_links_match_reference(lgt):
+    """The reference's localHeatbathUpdate, fed the RANLUX uniforms it consumed,
+    against the engine's FP64 per-link heatbath fed the same uniforms."""
+    fx = load("hb_links.npz")
+    links = np.zeros_like(fx["links"])
+    counts = np.zeros_like(fx["counts"])
+    for beta in np.unique(fx["beta"]):
+        sel = np.nonzero(fx["beta"] == beta)[0]
+        l, c = lgt.heatbath_harness(fx["staples"][sel], float(beta), fx["uniforms"], fx["offsets"][sel],
+                                    fx["lens"][sel])
+        links[sel], counts[sel] = l, c
+    np.testing.assert_array_equal(counts, fx["counts"])
+    u = ulp_diff(links, fx["links"])
+    exact = np.mean(np.all(links == fx["links"], axis=1))
+    print(f"heatbath per link: {exact * 100:.1f}% bit-identical, max {u.max():.0f} ulp")
+    assert np.max(np.abs(links - fx["links"])) <= 1e-14
+
+
+def test_heatbath_degenerate_branch(lgt):
+    fx = load("su2_random.npz")
+    n = len(fx["links"])
+    staples = np.zeros((n, 4))
+    lens = np.full(n, 3, dtype=np.int32)
+    offsets = np.arange(n, dtype=np.int32) * 3
+    links, counts = lgt.heatbath_harness(staples, 3.09, fx["uniforms"].ravel(), offsets, lens)
+    assert np.all(counts == 0)
+    assert np.max(np.abs(links - fx["links"])) <= 1e-15
+
+
+def test_overrelax_harness(lgt):
+    rng = np.random.default_rng(2)
+    n = 512
+    A = rng.normal(size=(n, 4)) * rng.uniform(0.1, 6, size=(n, 1))
+    U = F.haar(rng, (n,))
+    out = lgt.overrelax_harness(A, U)
+    import ctypes as C
+    lib = Oracle.lib()
+    P = C.POINTER(C.c_double)
+    ref = np.zeros_like(out)
+    for i in range(n):
+        a, u, r = A[i].copy(), U[i].copy(), np.zeros(4)
+        lib.or_or_link(a.ctypes.data_as(P), u.ctypes.data_as(P), r.ctypes.data_as(P))
+        ref[i] = r
+    np.testing.assert_array_equal(out, ref)
+
+
+@pytest.mark.parametrize("L", [(4, 4, 4, 4), (6, 4, 4, 8)])
+def test_heatbath_subsweep_replay(lgt, L):
+    """Each parallel (mu, parity) sub-sweep equals the oracle's sequential
+    replay with the same Philox streams (checkerboard independence)."""
+    seed = 99
+    o, U = thermal_field(L)
+    with engine(lgt, L, 0, seed=seed, U=U) as e:
+        V = U.copy()
+        for mu in range(4):
+            for p in range(2):
+                o.hb_subsweep(V, mu, p, 3.09, seed, 0)
+                e.heatbath_subsweep(mu, p, 3.09)
+                G = e.download()
+                d = np.max(np.abs(G - V))
+                assert d <= 1e-13, (mu, p, d)
+                V = G.copy()                 # continue from the engine's field
+
+
+@pytest.mark.parametrize("L", [(4, 4, 4, 4), (6, 4, 4, 8)])
+@pytest.mark.parametrize("n_or", [0, 1, 3])
+def test_update_matches_oracle_chain(lgt, L, n_or):
+    seed = 2024
+    o, U = thermal_field(L, seed=5, nsweeps=1)
+    V = U.copy()
+    with engine(lgt, L, 0, seed=seed, U=U) as e:
+        acc = e.update(2, n_or, 3.09, acceptance=True)
+        G = e.download()
+    acc_ref = 0.0
+    for s in range(2):
+        acc_ref += o.engine_sweep(V, 3.09, n_or, seed, s) / 2
+    d = np.max(np.abs(G - V))
+    print(f"update L={L} n_or={n_or}: max|dU| = {d:.3g}, acceptance {acc:.6f} vs {acc_ref:.6f}")
+    assert d <= 1e-12
+    assert acc == pytest.approx(acc_ref, rel=1e-12)
+    np.testing.assert_allclose(np.linalg.norm(G, axis=-1), 1.0, atol=1e-15)
+
+
+def test_update_is_deterministic_and_tile_order_free(lgt):
+    L = (8, 8, 8, 8)
+    res = []
+    for remap in (1, 0, 1):
+        with engine(lgt, L, 1, seed=7) as e:
+            e.set_option(0, remap)
+            e.init_field(1)
+            e.update(3, 1, 3.09)
+            res.append(e.download())
+    np.testing.assert_array_equal(res[0], res[1])
+    np.testing.assert_array_equal(res[0], res[2])
+
+
+# ---------------------------------------------------------------------------
+# FP32 storage against FP64
+# ---------------------------------------------------------------------------
+def test_fp32_observables_track_fp64(lgt):
+    L = (8, 8, 8, 8)
+    o, U = thermal_field(L)
+    with engine(lgt, L, 0, U=U) as e64, engine(lgt, L, 1, U=U) as e32:
+        assert e32.plaquette() == pytest.approx(e64.plaquette(), abs=2e-7)
+        e64.ape(0, 20, 0.25)
+        e32.ape(0, 20, 0.25)
+        d = np.max(np.abs(e32.download() - e64.download()))
+        assert d < 2e-5, d
+        R, T = [1, 2, 3, 4], [1, 2, 3, 4]
+        np.testing.assert_allclose(e32.wilson_loops(R, T), e64.wilson_loops(R, T), atol=2e-6)
+        e64.ape(1, 5, 1.0 / 6.0)
+        e32.ape(1, 5, 1.0 / 6.0)
+        q64, s64 = e64.qtop(0)
+        q32, s32 = e32.qtop(0)
+        assert abs(q32 - q64) < 1e-4 * s64
+
+
+def test_fp32_update_stays_unitary(lgt):
+    L = (8, 8, 8, 8)
+    with engine(lgt, L, 1, seed=3) as e:
+        e.init_field(1)
+        acc = e.update(5, 1, 3.09, acceptance=True)
+        U = e.download()
+        p = e.plaquette()
+    np.testing.assert_allclose(np.linalg.norm(U, axis=-1), 1.0, atol=3e-7)
+    assert 0.5 < acc <= 1.0
+    assert 0.3 < p < 0.8
