@@ -91,6 +91,21 @@ def time_ape(e, prec, L, nsteps=20):
     return {"steps_per_s": nsteps / dt, "kernel_ms_avg": avg, "achieved_GBs": gbs, "frac_hbm": gbs / HBM_PEAK_GBS}
 
 
+def time_stream(e, prec, L, reps=10):
+    """The projection kernel reads and writes every link once: a plain
+    streaming pass of known size, the achievable-bandwidth reference and the
+    calibration kernel of the HBM byte counters."""
+    vol = int(np.prod(L))
+    e.set_option(1, 1)
+    e.profile_reset()
+    for _ in range(reps):
+        e.project()
+    ms, n = e.profile_read(2)
+    e.set_option(1, 0)
+    nbytes = 2 * 4 * vol * (16 if prec == 1 else 32)
+    return {"bytes_per_launch": nbytes, "kernel_ms_avg": ms / n, "achieved_GBs": nbytes / (ms / n * 1e-3) / 1e9}
+
+
 def time_observables(e):
     out = {}
     t0 = time.perf_counter()
@@ -166,6 +181,8 @@ def main():
 
     e32, r32 = run_chain(lgt, L, lgt.FP32, args.seed, args.beta, args.n_or, args.warmup, args.steps, args.nplaq)
     log(f"[bench] fp32 sweep: {r32['ms_per_step']:.3f} ms/step, {r32['link_updates_per_s']:.4g} link updates/s")
+    stream32 = time_stream(e32, 1, L)
+    log(f"[bench] fp32 streaming pass (project): {stream32['achieved_GBs']:.0f} GB/s")
     ape32 = time_ape(e32, 1, L)
     log(f"[bench] fp32 spatial APE: {ape32['steps_per_s']:.1f} steps/s")
     obs32 = time_observables(e32)
@@ -242,6 +259,7 @@ def main():
             "n": len(p32),
         },
         "ape_spatial": {"fp32": ape32, "fp64": ape64},
+        "stream_pass": stream32,
         "observables_s": {"fp32": obs32, "fp64": obs64},
         "device_bytes": {"fp32": r32["device_bytes"], "fp64": r64["device_bytes"]},
     }

@@ -17,14 +17,25 @@ constexpr int BLOCK = 256;
 constexpr int NWAVES = BLOCK / 64;
 constexpr double PI_D = 3.14159265358979323846;   // = 2 asin(1), LGT_Code/modules/admin/init.c:63
 
-// Workgroups are dispatched round-robin over the 8 XCDs of the chip; with
-// remap on, XCD k processes the k-th contiguous eighth of the grid so that
-// neighbouring tiles share one L2.
-__device__ __forceinline__ uint32_t tile_id(int remap)
+// Tile order of the 256-site blocks of a checkerboard plane.
+//  bit 0: workgroups are dispatched round-robin over the 8 XCDs of the chip;
+//         XCD k takes the k-th contiguous eighth of the tile sequence, so that
+//         tiles that share neighbours share one L2.
+//  bit 1: time fastest: consecutive tiles are the same spatial tile at
+//         consecutive t, so the +-t neighbours (a whole time slice away in
+//         memory) are read while still in L2.  Needs the cb sites of one time
+//         slice to be a multiple of 256; the host clears the bit otherwise.
+// Returns the first cb index of the block's tile.
+__device__ __forceinline__ uint32_t tile_base(const Geo& g)
 {
     uint32_t b = blockIdx.x, nb = gridDim.x;
-    if (remap && (nb & 7u) == 0) b = (b & 7u) * (nb >> 3) + (b >> 3);
-    return b;
+    if ((g.order & 1) && (nb & 7u) == 0) b = (b & 7u) * (nb >> 3) + (b >> 3);
+    if (g.order & 2) {
+        uint32_t s = fdiv(g.dt, b);
+        uint32_t t = b - s * (uint32_t)g.L[0];
+        return t * (g.vh / (uint32_t)g.L[0]) + s * 256u;
+    }
+    return b * 256u;
 }
 
 // ---------------------------------------------------------------------------
@@ -328,10 +339,11 @@ template <> struct HB<float> {
 // the block's sum of 1/count (nullable).
 template <typename T, int MU>
 __global__ void __launch_bounds__(BLOCK) k_heatbath(Planes<T> P, Geo g, int p, T beta, uint64_t seed, uint32_t sweep,
-                                                    int remap, DD* __restrict__ acc_partial)
+                                                    DD* __restrict__ acc_partial)
 {
-    uint32_t tile = tile_id(remap);
-    uint32_t i = tile * BLOCK + threadIdx.x;
+    uint32_t base = tile_base(g);
+    uint32_t tile = base / BLOCK;
+    uint32_t i = base + threadIdx.x;
     double acc = 0.0;
     if (i < g.vh) {
         Site s = cb_site(g, i, p);
@@ -371,9 +383,9 @@ template <> __device__ __forceinline__ Q<float> overrelax_link(const Q<float>& A
 }
 
 template <typename T, int MU>
-__global__ void __launch_bounds__(BLOCK) k_overrelax(Planes<T> P, Geo g, int p, int remap)
+__global__ void __launch_bounds__(BLOCK) k_overrelax(Planes<T> P, Geo g, int p)
 {
-    uint32_t i = tile_id(remap) * BLOCK + threadIdx.x;
+    uint32_t i = tile_base(g) + threadIdx.x;
     if (i >= g.vh) return;
     Site s = cb_site(g, i, p);
     Q<T> A = update_staple<T, MU>(P, g, s, p);
@@ -429,10 +441,11 @@ __device__ __forceinline__ T plaq_term(const Planes<T>& P, const Geo& g, const S
 
 // grid: x over cb sites, y = parity; partial[y*gridDim.x + tile]
 template <typename T>
-__global__ void __launch_bounds__(BLOCK) k_plaquette(Planes<T> P, Geo g, int remap, DD* __restrict__ partial)
+__global__ void __launch_bounds__(BLOCK) k_plaquette(Planes<T> P, Geo g, DD* __restrict__ partial)
 {
-    uint32_t tile = tile_id(remap);
-    uint32_t i = tile * BLOCK + threadIdx.x;
+    uint32_t base = tile_base(g);
+    uint32_t tile = base / BLOCK;
+    uint32_t i = base + threadIdx.x;
     int p = blockIdx.y;
     DD acc = DD{0.0, 0.0};
     if (i < g.vh) {
@@ -476,9 +489,9 @@ __device__ __forceinline__ Q<T> ape_staple(const Planes<T>& Z, const Geo& g, con
 
 // grid: x over cb sites, y = (mu - mu0)*2 + parity
 template <typename T, int MODE>
-__global__ void __launch_bounds__(BLOCK) k_ape(Planes<T> Z, Planes<T> O, Geo g, T sp, int remap)
+__global__ void __launch_bounds__(BLOCK) k_ape(Planes<T> Z, Planes<T> O, Geo g, T sp)
 {
-    uint32_t i = tile_id(remap) * BLOCK + threadIdx.x;
+    uint32_t i = tile_base(g) + threadIdx.x;
     if (i >= g.vh) return;
     int p = blockIdx.y & 1;
     int mu = (MODE == 0 ? 1 : 0) + (blockIdx.y >> 1);
@@ -516,12 +529,13 @@ __device__ __forceinline__ Q<T> spatial_line(const Planes<T>& P, const Geo& g, S
 // grid: x over cb sites, y = parity, z = group*3 + (d-1)
 template <typename T>
 __global__ void __launch_bounds__(BLOCK) k_wloops(Planes<T> P, Geo g, const WGroup* __restrict__ groups,
-                                                  const int* __restrict__ slots, int remap, DD* __restrict__ partial,
+                                                  const int* __restrict__ slots, DD* __restrict__ partial,
                                                   int ncols)
 {
     __shared__ DD wacc[NWAVES][WMAXT];
-    uint32_t tile = tile_id(remap);
-    uint32_t i = tile * BLOCK + threadIdx.x;
+    uint32_t base = tile_base(g);
+    uint32_t tile = base / BLOCK;
+    uint32_t i = base + threadIdx.x;
     const int p = blockIdx.y;
     const WGroup grp = groups[blockIdx.z / 3];
     const int d = 1 + (int)(blockIdx.z % 3);
@@ -727,11 +741,12 @@ __device__ __forceinline__ T clover_products_1x1(const Planes<T>& P, const Geo& 
 // partial[k] = sum q and partial[ncols + k] = sum |q| with k = y*gridDim.x +
 // tile, ncols = 2*gridDim.x; qx: density in lexicographic order (nullable).
 template <typename T, int KIND>
-__global__ void __launch_bounds__(BLOCK) k_qtop(Planes<T> P, Geo g, int remap, DD* __restrict__ partial,
+__global__ void __launch_bounds__(BLOCK) k_qtop(Planes<T> P, Geo g, DD* __restrict__ partial,
                                                 double* __restrict__ qx)
 {
-    uint32_t tile = tile_id(remap);
-    uint32_t i = tile * BLOCK + threadIdx.x;
+    uint32_t base = tile_base(g);
+    uint32_t tile = base / BLOCK;
+    uint32_t i = base + threadIdx.x;
     int p = blockIdx.y;
     double q = 0.0;
     if (i < g.vh) {

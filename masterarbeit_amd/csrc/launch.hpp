@@ -26,17 +26,16 @@ template <typename T> struct Launch {
     static hipError_t project(hipStream_t s, const Geo& g, const RawPlanes& P);
     // acc_partial: nullable, receives ceil(vh/256) block sums of 1/count
     static hipError_t heatbath(hipStream_t s, const Geo& g, const RawPlanes& P, int mu, int parity, double beta,
-                               uint64_t seed, uint32_t sweep, int remap, void* acc_partial);
-    static hipError_t overrelax(hipStream_t s, const Geo& g, const RawPlanes& P, int mu, int parity, int remap);
+                               uint64_t seed, uint32_t sweep, void* acc_partial);
+    static hipError_t overrelax(hipStream_t s, const Geo& g, const RawPlanes& P, int mu, int parity);
     // partial: 2*ceil(vh/256) double-doubles
-    static hipError_t plaquette(hipStream_t s, const Geo& g, const RawPlanes& P, int remap, void* partial);
-    static hipError_t ape(hipStream_t s, const Geo& g, const RawPlanes& in, const RawPlanes& out, int mode, double sp,
-                          int remap);
+    static hipError_t plaquette(hipStream_t s, const Geo& g, const RawPlanes& P, void* partial);
+    static hipError_t ape(hipStream_t s, const Geo& g, const RawPlanes& in, const RawPlanes& out, int mode, double sp);
     // groups/slots: device tables; partial: nloop rows of 6*ceil(vh/256) columns
     static hipError_t wloops(hipStream_t s, const Geo& g, const RawPlanes& P, const WGroupHost* groups, int ngroups,
-                             const int* slots, int remap, void* partial, int ncols);
+                             const int* slots, void* partial, int ncols);
     // partial: 2 rows of 2*ceil(vh/256) columns (sum q, sum |q|)
-    static hipError_t qtop(hipStream_t s, const Geo& g, const RawPlanes& P, int kind, int remap, void* partial,
+    static hipError_t qtop(hipStream_t s, const Geo& g, const RawPlanes& P, int kind, void* partial,
                            double* density);
 };
 

@@ -52,71 +52,70 @@ hipError_t Launch<T>::project(hipStream_t s, const Geo& g, const RawPlanes& P)
 
 template <typename T>
 hipError_t Launch<T>::heatbath(hipStream_t s, const Geo& g, const RawPlanes& P, int mu, int parity, double beta,
-                               uint64_t seed, uint32_t sweep, int remap, void* acc_partial)
+                               uint64_t seed, uint32_t sweep, void* acc_partial)
 {
     dim3 grid(grid_x(g)), block(BLOCK);
     Planes<T> Pl = planes<T>(P);
     DD* ap = (DD*)acc_partial;
     switch (mu) {
-    case 0: hipLaunchKernelGGL((k_heatbath<T, 0>), grid, block, 0, s, Pl, g, parity, (T)beta, seed, sweep, remap, ap); break;
-    case 1: hipLaunchKernelGGL((k_heatbath<T, 1>), grid, block, 0, s, Pl, g, parity, (T)beta, seed, sweep, remap, ap); break;
-    case 2: hipLaunchKernelGGL((k_heatbath<T, 2>), grid, block, 0, s, Pl, g, parity, (T)beta, seed, sweep, remap, ap); break;
-    default: hipLaunchKernelGGL((k_heatbath<T, 3>), grid, block, 0, s, Pl, g, parity, (T)beta, seed, sweep, remap, ap); break;
+    case 0: hipLaunchKernelGGL((k_heatbath<T, 0>), grid, block, 0, s, Pl, g, parity, (T)beta, seed, sweep, ap); break;
+    case 1: hipLaunchKernelGGL((k_heatbath<T, 1>), grid, block, 0, s, Pl, g, parity, (T)beta, seed, sweep, ap); break;
+    case 2: hipLaunchKernelGGL((k_heatbath<T, 2>), grid, block, 0, s, Pl, g, parity, (T)beta, seed, sweep, ap); break;
+    default: hipLaunchKernelGGL((k_heatbath<T, 3>), grid, block, 0, s, Pl, g, parity, (T)beta, seed, sweep, ap); break;
     }
     return hipGetLastError();
 }
 
 template <typename T>
-hipError_t Launch<T>::overrelax(hipStream_t s, const Geo& g, const RawPlanes& P, int mu, int parity, int remap)
+hipError_t Launch<T>::overrelax(hipStream_t s, const Geo& g, const RawPlanes& P, int mu, int parity)
 {
     dim3 grid(grid_x(g)), block(BLOCK);
     Planes<T> Pl = planes<T>(P);
     switch (mu) {
-    case 0: hipLaunchKernelGGL((k_overrelax<T, 0>), grid, block, 0, s, Pl, g, parity, remap); break;
-    case 1: hipLaunchKernelGGL((k_overrelax<T, 1>), grid, block, 0, s, Pl, g, parity, remap); break;
-    case 2: hipLaunchKernelGGL((k_overrelax<T, 2>), grid, block, 0, s, Pl, g, parity, remap); break;
-    default: hipLaunchKernelGGL((k_overrelax<T, 3>), grid, block, 0, s, Pl, g, parity, remap); break;
+    case 0: hipLaunchKernelGGL((k_overrelax<T, 0>), grid, block, 0, s, Pl, g, parity); break;
+    case 1: hipLaunchKernelGGL((k_overrelax<T, 1>), grid, block, 0, s, Pl, g, parity); break;
+    case 2: hipLaunchKernelGGL((k_overrelax<T, 2>), grid, block, 0, s, Pl, g, parity); break;
+    default: hipLaunchKernelGGL((k_overrelax<T, 3>), grid, block, 0, s, Pl, g, parity); break;
     }
     return hipGetLastError();
 }
 
 template <typename T>
-hipError_t Launch<T>::plaquette(hipStream_t s, const Geo& g, const RawPlanes& P, int remap, void* partial)
+hipError_t Launch<T>::plaquette(hipStream_t s, const Geo& g, const RawPlanes& P, void* partial)
 {
-    hipLaunchKernelGGL(k_plaquette<T>, dim3(grid_x(g), 2), dim3(BLOCK), 0, s, planes<T>(P), g, remap, (DD*)partial);
+    hipLaunchKernelGGL(k_plaquette<T>, dim3(grid_x(g), 2), dim3(BLOCK), 0, s, planes<T>(P), g, (DD*)partial);
     return hipGetLastError();
 }
 
 template <typename T>
-hipError_t Launch<T>::ape(hipStream_t s, const Geo& g, const RawPlanes& in, const RawPlanes& out, int mode, double sp,
-                          int remap)
+hipError_t Launch<T>::ape(hipStream_t s, const Geo& g, const RawPlanes& in, const RawPlanes& out, int mode, double sp)
 {
     int nmu = (mode == 0) ? 3 : (mode == 1 ? 4 : 1);
     dim3 grid(grid_x(g), 2 * nmu), block(BLOCK);
     Planes<T> Z = planes<T>(in), O = planes<T>(out);
-    if (mode == 0) hipLaunchKernelGGL((k_ape<T, 0>), grid, block, 0, s, Z, O, g, (T)sp, remap);
-    else if (mode == 1) hipLaunchKernelGGL((k_ape<T, 1>), grid, block, 0, s, Z, O, g, (T)sp, remap);
-    else hipLaunchKernelGGL((k_ape<T, 2>), grid, block, 0, s, Z, O, g, (T)sp, remap);
+    if (mode == 0) hipLaunchKernelGGL((k_ape<T, 0>), grid, block, 0, s, Z, O, g, (T)sp);
+    else if (mode == 1) hipLaunchKernelGGL((k_ape<T, 1>), grid, block, 0, s, Z, O, g, (T)sp);
+    else hipLaunchKernelGGL((k_ape<T, 2>), grid, block, 0, s, Z, O, g, (T)sp);
     return hipGetLastError();
 }
 
 template <typename T>
 hipError_t Launch<T>::wloops(hipStream_t s, const Geo& g, const RawPlanes& P, const WGroupHost* groups, int ngroups,
-                             const int* slots, int remap, void* partial, int ncols)
+                             const int* slots, void* partial, int ncols)
 {
     dim3 grid(grid_x(g), 2, 3 * ngroups), block(BLOCK);
-    hipLaunchKernelGGL(k_wloops<T>, grid, block, 0, s, planes<T>(P), g, (const WGroup*)groups, slots, remap,
+    hipLaunchKernelGGL(k_wloops<T>, grid, block, 0, s, planes<T>(P), g, (const WGroup*)groups, slots,
                        (DD*)partial, ncols);
     return hipGetLastError();
 }
 
 template <typename T>
-hipError_t Launch<T>::qtop(hipStream_t s, const Geo& g, const RawPlanes& P, int kind, int remap, void* partial,
+hipError_t Launch<T>::qtop(hipStream_t s, const Geo& g, const RawPlanes& P, int kind, void* partial,
                            double* density)
 {
     dim3 grid(grid_x(g), 2), block(BLOCK);
-    if (kind == 0) hipLaunchKernelGGL((k_qtop<T, 0>), grid, block, 0, s, planes<T>(P), g, remap, (DD*)partial, density);
-    else hipLaunchKernelGGL((k_qtop<T, 1>), grid, block, 0, s, planes<T>(P), g, remap, (DD*)partial, density);
+    if (kind == 0) hipLaunchKernelGGL((k_qtop<T, 0>), grid, block, 0, s, planes<T>(P), g, (DD*)partial, density);
+    else hipLaunchKernelGGL((k_qtop<T, 1>), grid, block, 0, s, planes<T>(P), g, (DD*)partial, density);
     return hipGetLastError();
 }
 

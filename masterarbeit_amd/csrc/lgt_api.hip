@@ -70,7 +70,6 @@ struct lgt_engine {
     size_t hres_bytes = 0;
     void* staging = nullptr;
     size_t staging_bytes = 0;
-    int remap = 1;
     int profile = 0;
     std::vector<ProfEvent> events;
     double prof_ms[LGT_K_COUNT] = {0};
@@ -170,6 +169,13 @@ int flush_profile(lgt_engine* e)
 
 uint32_t nbx(const lgt_engine* e) { return grid_x(e->geo); }
 
+// the time-fastest order needs whole 256-site tiles per time slice
+int tile_order_ok(const Geo& g, int order)
+{
+    if ((order & 2) && ((g.vh / (uint32_t)g.L[0]) % 256u) != 0) order &= ~2;
+    return order;
+}
+
 // reduces `nrows` rows of `ncols` double-doubles to host doubles (hi + lo)
 int reduce_to_host(lgt_engine* e, int nrows, int ncols, double* out)
 {
@@ -235,6 +241,8 @@ int lgt_create(const int extents[4], int precision, uint64_t seed, lgt_engine** 
     g.dz = make_fastdiv((uint32_t)g.hz);
     g.dy = make_fastdiv((uint32_t)extents[2]);
     g.dx = make_fastdiv((uint32_t)extents[1]);
+    g.dt = make_fastdiv((uint32_t)extents[0]);
+    g.order = tile_order_ok(g, 3);
     e->link_bytes = (precision == LGT_FP64) ? 32 : 16;
 
     err = hipStreamCreateWithFlags(&e->stream, hipStreamNonBlocking);
@@ -284,7 +292,10 @@ int lgt_set_option(lgt_engine* e, int option, int value)
 {
     if (!e) return fail(LGT_EINVAL, "lgt_set_option: null engine");
     switch (option) {
-    case LGT_OPT_XCD_REMAP: e->remap = value ? 1 : 0; return LGT_OK;
+    case LGT_OPT_TILE_ORDER:
+        if (value < 0 || value > 3) return fail(LGT_EINVAL, "lgt_set_option: tile order %d", value);
+        e->geo.order = tile_order_ok(e->geo, value);
+        return LGT_OK;
     case LGT_OPT_PROFILE:
         if (!value) TRY(flush_profile(e));
         e->profile = value ? 1 : 0;
@@ -348,7 +359,7 @@ int lgt_heatbath_subsweep(lgt_engine* e, int mu, int parity, double beta, double
         ap = e->partial;
     }
     TRY(run(e, LGT_K_HEATBATH, "heatbath", [&] {
-        return DISPATCH(e, heatbath(e->stream, e->geo, e->cur, mu, parity, beta, e->seed, e->sweep, e->remap, ap));
+        return DISPATCH(e, heatbath(e->stream, e->geo, e->cur, mu, parity, beta, e->seed, e->sweep, ap));
     }));
     if (acc_sum) TRY(reduce_to_host(e, 1, (int)nbx(e), acc_sum));
     return LGT_OK;
@@ -359,7 +370,7 @@ int lgt_overrelax_subsweep(lgt_engine* e, int mu, int parity)
     if (!e || mu < 0 || mu > 3 || parity < 0 || parity > 1) return fail(LGT_EINVAL, "lgt_overrelax_subsweep: mu/parity");
     TRY(set_dev(e));
     return run(e, LGT_K_OVERRELAX, "overrelax",
-               [&] { return DISPATCH(e, overrelax(e->stream, e->geo, e->cur, mu, parity, e->remap)); });
+               [&] { return DISPATCH(e, overrelax(e->stream, e->geo, e->cur, mu, parity)); });
 }
 
 int lgt_project(lgt_engine* e)
@@ -394,7 +405,7 @@ int lgt_update(lgt_engine* e, int nsweeps, int n_or, double beta, double* accept
             for (int p = 0; p < 2; p++) {
                 void* ap = acceptance ? (char*)e->partial + sizeof(double) * 2 * (size_t)nb * (mu * 2 + p) : nullptr;
                 TRY(run(e, LGT_K_HEATBATH, "heatbath", [&] {
-                    return DISPATCH(e, heatbath(e->stream, e->geo, e->cur, mu, p, beta, e->seed, e->sweep, e->remap, ap));
+                    return DISPATCH(e, heatbath(e->stream, e->geo, e->cur, mu, p, beta, e->seed, e->sweep, ap));
                 }));
             }
         if (acceptance) TRY(reduce_to_host(e, 1, (int)(8 * nb), &acc_rows[s]));
@@ -402,7 +413,7 @@ int lgt_update(lgt_engine* e, int nsweeps, int n_or, double beta, double* accept
             for (int mu = 0; mu < 4; mu++)
                 for (int p = 0; p < 2; p++)
                     TRY(run(e, LGT_K_OVERRELAX, "overrelax",
-                            [&] { return DISPATCH(e, overrelax(e->stream, e->geo, e->cur, mu, p, e->remap)); }));
+                            [&] { return DISPATCH(e, overrelax(e->stream, e->geo, e->cur, mu, p)); }));
         if (n_or == 0) TRY(run(e, LGT_K_PROJECT, "project", [&] { return DISPATCH(e, project(e->stream, e->geo, e->cur)); }));
         e->sweep++;
     }
@@ -422,7 +433,7 @@ int lgt_plaquette(lgt_engine* e, double* plaq)
     const uint32_t nb = nbx(e);
     TRY(ensure(e, &e->partial, &e->partial_bytes, sizeof(double) * 2 * 2 * (size_t)nb, "partials"));
     TRY(run(e, LGT_K_PLAQUETTE, "plaquette",
-            [&] { return DISPATCH(e, plaquette(e->stream, e->geo, e->cur, e->remap, e->partial)); }));
+            [&] { return DISPATCH(e, plaquette(e->stream, e->geo, e->cur, e->partial)); }));
     double s = 0.0;
     TRY(reduce_to_host(e, 1, (int)(2 * nb), &s));
     *plaq = s / (double)(2 * 6 * e->vol);
@@ -451,7 +462,7 @@ int lgt_ape(lgt_engine* e, int mode, int nsteps, double sp)
     }
     int mu_lo = (mode == LGT_APE_SPATIAL) ? 1 : 0, mu_hi = (mode == LGT_APE_TEMPORAL) ? 0 : 3;
     for (int s = 0; s < nsteps; s++) {
-        TRY(run(e, LGT_K_APE, "ape", [&] { return DISPATCH(e, ape(e->stream, e->geo, e->cur, e->alt, mode, sp, e->remap)); }));
+        TRY(run(e, LGT_K_APE, "ape", [&] { return DISPATCH(e, ape(e->stream, e->geo, e->cur, e->alt, mode, sp)); }));
         for (int mu = mu_lo; mu <= mu_hi; mu++)
             for (int p = 0; p < 2; p++) std::swap(e->cur.p[mu][p], e->alt.p[mu][p]);
     }
@@ -507,7 +518,7 @@ int lgt_wilson_loops(lgt_engine* e, int nloop, const int* R, const int* T, doubl
     HIP_TRY(hipMemcpyAsync(dslots, slots.data(), sizeof(int) * slots.size(), hipMemcpyHostToDevice, e->stream),
             "hipMemcpyAsync");
     TRY(run(e, LGT_K_WLOOP, "wilson loops", [&] {
-        return DISPATCH(e, wloops(e->stream, e->geo, e->cur, dgroups, (int)groups.size(), dslots, e->remap, e->partial, ncols));
+        return DISPATCH(e, wloops(e->stream, e->geo, e->cur, dgroups, (int)groups.size(), dslots, e->partial, ncols));
     }));
     std::vector<double> sums(nuniq);
     TRY(reduce_to_host(e, nuniq, ncols, sums.data()));
@@ -527,7 +538,7 @@ int lgt_qtop(lgt_engine* e, int kind, double* q, double* abs_sum, double* densit
         dq = (double*)e->staging;
     }
     TRY(run(e, LGT_K_QTOP, "qtop",
-            [&] { return DISPATCH(e, qtop(e->stream, e->geo, e->cur, kind, e->remap, e->partial, dq)); }));
+            [&] { return DISPATCH(e, qtop(e->stream, e->geo, e->cur, kind, e->partial, dq)); }));
     double s[2];
     TRY(reduce_to_host(e, 2, (int)(2 * nb), s));
     const double PI = 3.14159265358979323846;

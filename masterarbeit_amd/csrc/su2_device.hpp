@@ -154,6 +154,8 @@ struct Geo {
     uint32_t vh;       // VOL / 2
     uint32_t stride[4];// cb index strides of t, x, y (z handled apart)
     FastDiv dz, dy, dx;// divisions by hz, L2, L1
+    FastDiv dt;        // division by L0 (tile order)
+    int order;         // tile order flags: 1 = XCD-contiguous, 2 = time fastest
 };
 
 struct Site { int c[4]; };

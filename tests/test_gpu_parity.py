@@ -289,17 +289,22 @@ def test_update_matches_oracle_chain(lgt, L, n_or):
     np.testing.assert_allclose(np.linalg.norm(G, axis=-1), 1.0, atol=1e-15)
 
 
-def test_update_is_deterministic_and_tile_order_free(lgt):
-    L = (8, 8, 8, 8)
+@pytest.mark.parametrize("prec", [0, 1])
+def test_results_independent_of_tile_order(lgt, prec):
+    """Every tile order (XCD-contiguous, time-fastest, both, neither) gives
+    bit-identical sweeps, smearing and observables; repeated runs too."""
+    L = (8, 8, 8, 8)       # one time slice = 256 cb sites: the time-fastest order applies
     res = []
-    for remap in (1, 0, 1):
-        with engine(lgt, L, 1, seed=7) as e:
-            e.set_option(0, remap)
+    for order in (3, 0, 1, 2, 3):
+        with engine(lgt, L, prec, seed=7) as e:
+            e.set_option(0, order)
             e.init_field(1)
             e.update(3, 1, 3.09)
-            res.append(e.download())
-    np.testing.assert_array_equal(res[0], res[1])
-    np.testing.assert_array_equal(res[0], res[2])
+            e.ape(0, 2, 0.25)
+            res.append((e.download(), e.plaquette(), e.wilson_loops([1, 2], [2, 3]).tolist(), e.qtop(0)))
+    for r in res[1:]:
+        np.testing.assert_array_equal(r[0], res[0][0])
+        assert r[1:] == res[0][1:]
 
 
 # ---------------------------------------------------------------------------
