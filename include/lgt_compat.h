@@ -1,0 +1,96 @@
+/*
+ * lgt_compat.h -- the reference's own C entry points for the SU(2) hot path,
+ * exported by masterarbeit_amd/liblgt_compat.so on top of the engine
+ * (lgt_su2.h).  A program built against the reference (LGT_Code/main/qcd.c,
+ * main_meas/meas.c, main_qtop/qtop.c) links this library instead of the
+ * reference's update.c, heatbath.c, metro.c, plaquette.c, smearing.c,
+ * wilson.c and topologicalcharge.c and runs unchanged on one GPU.
+ *
+ * The reference keeps its state in globals defined by the main program
+ * (LGT_Code/include/headers.h:376-413: pu, latParams, runParams, measParams)
+ * and reports errors through error(), which logs and exits
+ * (LGT_Code/modules/io/IO_utils.c:61-80).  This library reads those globals
+ * and calls the program's logging(), error(), getTime() and ranlxd(), so the
+ * declarations below repeat the reference's struct layouts member for member
+ * (DIM = 4, SUN = 2); only their layout matters, not these names' origin.
+ *
+ * Coherence: the host field (pu) is the authority between calls.  Every entry
+ * point uploads the field it reads and downloads the field it changes, so a
+ * caller that reads pu directly (writeConfig) always sees current links.
+ *
+ * Environment: LGT_PRECISION=fp64|fp32 (default fp64), LGT_N_OR=<n> (number
+ * of overrelaxation sweeps after each heatbath sweep, default 0 as in the
+ * reference).  The engine's RNG seed is drawn from the program's own ranlxd
+ * stream when the engine is created (first call).
+ */
+#ifndef LGT_COMPAT_H
+#define LGT_COMPAT_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- layouts of the reference's types (include/gauge.h:21-24, headers.h:345-374) ---- */
+typedef struct { double c0, c1, c2, c3; } su2mat;
+
+typedef struct {
+    int idForOutputFilesName, numConfs, numThermConfs, decorSteps, writeConfsFreq, numSweeps;
+    double beta, eps, mass;
+    int mwil, mcorrs, mtopcharge;
+} runParameters;
+
+typedef struct {
+    int linearExtent[4];
+    int volumeOtherDirs[4];
+    int volumeFactor[4];
+} latticeParameters;
+
+typedef struct {
+    int imeas;
+    int ts, tf, dt;
+    int rs, rf, dr;
+    int mwil_mode;
+    int *tExtents, *rExtents;
+    int ismt, isms, ism_tc;
+    double smpart, smpars, smpar_tc;
+    int nmax;
+    double eps;
+    int stype;
+    int source[4];
+    int n2pt;
+    int *g1, *g2;
+} measParameters;
+
+/* ---- globals and services the host program provides ---- */
+extern su2mat *pu[][4];                 /* headers.h:402 (sun_mat *pu[VOL][DIM]) */
+extern runParameters runParams;         /* headers.h:382 */
+extern latticeParameters latParams;     /* headers.h:383, set by initArrayOfNeighbours */
+extern measParameters measParams;       /* headers.h:384 */
+extern void logging(char *format, ...);                   /* IO_utils.c:47-59 */
+extern void error(int test, char *name, char *format, ...); /* IO_utils.c:61-80 */
+extern double getTime(void);                              /* utils.c:56-78 */
+extern void ranlxd(double r[], int n);                    /* ranlxd.c:571-586 */
+
+/* ---- exported entry points (LGT_Code/include/modules.h) ---- */
+/* modules.h:99, update.c:102-176: utype must be 1 (heatbath); the GPU sweep is
+ * checkerboard whatever stype says. */
+void gaugefieldUpdate(int iup, int nup, int utype, int stype);
+double plaquette(void);                                   /* modules.h:27, plaquette.c:67-101 */
+double gaugeAction(void);                                 /* modules.h:28, plaquette.c:104-110 */
+void smearing_APE_all(int ns, double sp, su2mat *u[][4]);      /* modules.h:32, smearing.c:74-105 */
+void smearing_APE_spatial(int ns, double sp, su2mat *u[][4]);  /* modules.h:33, smearing.c:107-138 */
+void smearing_APE_temporal(int ns, double sp, su2mat *u[][4]); /* modules.h:34, smearing.c:140-168 */
+void measureWilsonLoop(double *w);                        /* modules.h:38, wilson.c:268-324 */
+double topologicalCharge(void);                           /* modules.h:42, topologicalcharge.c:164-199 */
+void meas_topologicalcharge(double *q);                   /* modules.h:43, topologicalcharge.c:391-409 */
+void project_gfield_to_sun(su2mat *u[][4]);               /* modules.h:176, sun_vfunc.c:82-97 */
+
+/* no reference counterpart: releases the engine (also registered with atexit) */
+void lgt_compat_release(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

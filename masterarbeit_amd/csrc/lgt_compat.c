@@ -1,0 +1,170 @@
+/*
+ * lgt_compat.c -- the reference's entry points (include/lgt_compat.h) on top of
+ * the GPU engine (include/lgt_su2.h).  C host code; every call gathers the
+ * field through the program's pointer table, runs on the device and scatters
+ * changed links back, so pu stays the authority between calls.
+ */
+#include "../../include/lgt_compat.h"
+
+#include <math.h>
+#include <stdint.h>
+#include <stdlib.h>
+#include <string.h>
+
+#include "../../include/lgt_su2.h"
+
+static lgt_engine *g_eng = NULL;
+static int g_L[4];
+static int64_t g_vol = 0;
+static double *g_buf = NULL;
+static int g_nor = 0;
+
+static char NAME[] = "lgt_compat [lgt_compat.c]";
+
+static void check(int rc, const char *what)
+{
+    if (rc != LGT_OK) {
+        char fmt[] = "%s failed: %s (%s)";
+        error(1, NAME, fmt, what, lgt_strerror(rc), lgt_last_error());
+    }
+}
+
+void lgt_compat_release(void)
+{
+    if (g_eng) lgt_destroy(g_eng);
+    g_eng = NULL;
+    free(g_buf);
+    g_buf = NULL;
+}
+
+static void engine(void)
+{
+    if (g_eng) return;
+    g_vol = 1;
+    for (int d = 0; d < 4; d++) {
+        g_L[d] = latParams.linearExtent[d];
+        g_vol *= g_L[d];
+    }
+    if (g_vol <= 0) {
+        char fmt[] = "lattice extents unset: call initArrayOfNeighbours() first";
+        error(1, NAME, fmt);
+    }
+    int prec = LGT_FP64;
+    const char *p = getenv("LGT_PRECISION");
+    if (p && (!strcmp(p, "fp32") || !strcmp(p, "FP32"))) prec = LGT_FP32;
+    const char *n = getenv("LGT_N_OR");
+    g_nor = n ? atoi(n) : 0;
+    if (g_nor < 0) g_nor = 0;
+    /* seed from the program's own RANLUX stream (rlxd_init(2, seed) in main) */
+    double r[2];
+    ranlxd(r, 2);
+    uint64_t seed = ((uint64_t)(r[0] * 4294967296.0) << 32) | (uint64_t)(r[1] * 4294967296.0);
+    check(lgt_create(g_L, prec, seed, &g_eng), "lgt_create");
+    g_buf = (double *)malloc(sizeof(double) * 16 * (size_t)g_vol);
+    if (!g_buf) {
+        char fmt[] = "cannot allocate the %lld-site transfer buffer";
+        error(1, NAME, fmt, (long long)g_vol);
+    }
+    atexit(lgt_compat_release);
+}
+
+static void upload(su2mat *u[][4])
+{
+    engine();
+    for (int64_t s = 0; s < g_vol; s++)
+        for (int d = 0; d < 4; d++) memcpy(g_buf + 16 * s + 4 * d, u[s][d], 4 * sizeof(double));
+    check(lgt_upload(g_eng, g_buf), "lgt_upload");
+}
+
+static void download(su2mat *u[][4])
+{
+    check(lgt_download(g_eng, g_buf), "lgt_download");
+    for (int64_t s = 0; s < g_vol; s++)
+        for (int d = 0; d < 4; d++) memcpy(u[s][d], g_buf + 16 * s + 4 * d, 4 * sizeof(double));
+}
+
+void gaugefieldUpdate(int iup, int nup, int utype, int stype)
+{
+    char m1[] = "Only the heatbath (SIM_TYPE=1) runs on the GPU engine!";
+    char m2[] = "Unknown sweep type! Only sequential (SWEEP_TYPE=0) and random (SWEEP_TYPE=1) sweeps implemented!";
+    error(utype != 1, NAME, m1);
+    error(stype != 0 && stype != 1, NAME, m2);
+    double t1 = getTime();
+    upload(pu);
+    double acc = 0.0;
+    check(lgt_update(g_eng, nup, g_nor, runParams.beta, &acc), "lgt_update");
+    download(pu);
+    double t2 = getTime();
+    char fmt[] = "\nUpdate %d done:\n(algorithm %d; sweep %d; acc=%.3f)\nupdate took %.9f seconds\n";
+    logging(fmt, iup, utype, stype, acc, t2 - t1);
+}
+
+double plaquette(void)
+{
+    double p = 0.0;
+    upload(pu);
+    check(lgt_plaquette(g_eng, &p), "lgt_plaquette");
+    return p;
+}
+
+double gaugeAction(void)
+{
+    double p = plaquette();
+    return runParams.beta * 6 * (double)g_vol * (1. - p);
+}
+
+static void ape(int mode, int ns, double sp, su2mat *u[][4])
+{
+    upload(u);
+    check(lgt_ape(g_eng, mode, ns, sp), "lgt_ape");
+    download(u);
+}
+
+void smearing_APE_all(int ns, double sp, su2mat *u[][4]) { ape(LGT_APE_ALL, ns, sp, u); }
+void smearing_APE_spatial(int ns, double sp, su2mat *u[][4]) { ape(LGT_APE_SPATIAL, ns, sp, u); }
+void smearing_APE_temporal(int ns, double sp, su2mat *u[][4]) { ape(LGT_APE_TEMPORAL, ns, sp, u); }
+
+/* loop list as measureWilsonLoop builds it (wilson.c:274-285) */
+void measureWilsonLoop(double *w)
+{
+    int nloop;
+    if (measParams.mwil_mode <= 0) {
+        int nt = ((measParams.tf - measParams.ts) / measParams.dt) + 1;
+        int nr = ((measParams.rf - measParams.rs) / measParams.dr) + 1;
+        nloop = nt * nr;
+    } else {
+        nloop = measParams.mwil_mode;
+    }
+    for (int i = 0; i < nloop; i++) w[i] = 0.;
+    engine();
+    char m[] = "Lattice too small to measure loops!";
+    char nm[] = "meas_wilson";
+    error(latParams.linearExtent[0] - 1 < measParams.tf, nm, m);
+    upload(pu);
+    check(lgt_wilson_loops(g_eng, nloop, measParams.rExtents, measParams.tExtents, w), "lgt_wilson_loops");
+}
+
+double topologicalCharge(void)
+{
+    char m1[] = "calculating topological charge ..\n";
+    char m2[] = "measurement done (took %.3e sec)\n";
+    double t1 = getTime(), q = 0.0;
+    logging(m1);
+    upload(pu);
+    check(lgt_qtop(g_eng, LGT_Q_IMPROVED, &q, NULL, NULL), "lgt_qtop");
+    logging(m2, getTime() - t1);
+    return q;
+}
+
+void meas_topologicalcharge(double *q)
+{
+    upload(pu);
+    check(lgt_qtop(g_eng, LGT_Q_CLOVER, q, NULL, NULL), "lgt_qtop");
+}
+
+void project_gfield_to_sun(su2mat *u[][4])
+{
+    upload(u);
+    check(lgt_project(g_eng), "lgt_project");
+    download(u);
+}

@@ -1,0 +1,133 @@
+/*
+ * compat_driver.c -- a reference-style host program (the role of
+ * LGT_Code/main/qcd.c and main_meas/meas.c) that owns the globals and the
+ * logging/error/timer/RNG services and calls the reference's entry points,
+ * which liblgt_compat.so serves from the GPU engine.  Test infrastructure.
+ *
+ * usage: compat_driver L0 L1 L2 L3 field.bin outdir
+ */
+#include <stdarg.h>
+#include <stdio.h>
+#include <stdlib.h>
+#include <string.h>
+#include <sys/time.h>
+
+#include "../../include/lgt_compat.h"
+
+#define MAXVOL (16 * 16 * 16 * 16)
+
+/* the globals of a reference main program (headers.h:376-413 with MAIN_C) */
+su2mat *pu[MAXVOL][4];
+runParameters runParams;
+latticeParameters latParams;
+measParameters measParams;
+static char LOG_FILE[512];
+
+void logging(char *format, ...)
+{
+    FILE *f = fopen(LOG_FILE, "ab");
+    va_list ap;
+    va_start(ap, format);
+    vfprintf(f, format, ap);
+    va_end(ap);
+    fclose(f);
+}
+
+void error(int test, char *name, char *format, ...)
+{
+    if (!test) return;
+    va_list ap;
+    fprintf(stderr, "\nError in %s:\n", name);
+    va_start(ap, format);
+    vfprintf(stderr, format, ap);
+    va_end(ap);
+    fprintf(stderr, "\nProgram aborted!!!\n");
+    exit(-1);
+}
+
+double getTime(void)
+{
+    struct timeval t;
+    gettimeofday(&t, 0);
+    return t.tv_sec + 1e-6 * t.tv_usec;
+}
+
+/* deterministic stand-in for the program's RANLUX stream: 0.25, 0.5, 0.75, ... */
+void ranlxd(double r[], int n)
+{
+    static int k = 0;
+    for (int i = 0; i < n; i++) r[i] = 0.25 * (double)((k++ % 3) + 1);
+}
+
+static void write_field(const char *dir, const char *name, const double *buf, long vol)
+{
+    char path[1024];
+    snprintf(path, sizeof(path), "%s/%s", dir, name);
+    FILE *f = fopen(path, "wb");
+    fwrite(buf, sizeof(double), 16 * (size_t)vol, f);
+    fclose(f);
+}
+
+int main(int argc, char **argv)
+{
+    if (argc != 7) {
+        fprintf(stderr, "usage: %s L0 L1 L2 L3 field.bin outdir\n", argv[0]);
+        return 2;
+    }
+    long vol = 1;
+    for (int d = 0; d < 4; d++) {
+        latParams.linearExtent[d] = atoi(argv[1 + d]);
+        vol *= latParams.linearExtent[d];
+    }
+    if (vol > MAXVOL) return 2;
+    const char *out = argv[6];
+    snprintf(LOG_FILE, sizeof(LOG_FILE), "%s/log.txt", out);
+    double *field = malloc(sizeof(double) * 16 * vol), *orig = malloc(sizeof(double) * 16 * vol);
+    FILE *f = fopen(argv[5], "rb");
+    if (!f || fread(orig, sizeof(double), 16 * vol, f) != (size_t)(16 * vol)) return 3;
+    fclose(f);
+    memcpy(field, orig, sizeof(double) * 16 * vol);
+    for (long n = 0; n < vol; n++)
+        for (int d = 0; d < 4; d++) pu[n][d] = (su2mat *)(field + 16 * n + 4 * d);
+    runParams.beta = 3.09;
+
+    char path[1024];
+    snprintf(path, sizeof(path), "%s/summary.txt", out);
+    FILE *s = fopen(path, "w");
+    fprintf(s, "plaq %.17g\n", plaquette());
+    fprintf(s, "action %.17g\n", gaugeAction());
+
+    /* loop grid as readInputFileForMeas builds it: R outer, T inner (inp_IO.c:319-334) */
+    measParams.ts = 1; measParams.tf = 3; measParams.dt = 1;
+    measParams.rs = 1; measParams.rf = 2; measParams.dr = 1;
+    measParams.mwil_mode = 0;
+    int nloop = 6, R[6], T[6], k = 0;
+    for (int r = 1; r <= 2; r++)
+        for (int t = 1; t <= 3; t++) { R[k] = r; T[k] = t; k++; }
+    measParams.rExtents = R;
+    measParams.tExtents = T;
+    double w[6];
+    measureWilsonLoop(w);
+    for (int i = 0; i < nloop; i++) fprintf(s, "WIL %d %d\t%.17g\n", R[i], T[i], w[i]);
+    fprintf(s, "QTOP %.17g\n", topologicalCharge());
+    double qc;
+    meas_topologicalcharge(&qc);
+    fprintf(s, "QCLOV %.17g\n", qc);
+
+    smearing_APE_spatial(3, 0.25, pu);
+    write_field(out, "ape_spatial.bin", field, vol);
+    smearing_APE_all(2, 1.0 / 6.0, pu);
+    write_field(out, "ape_all.bin", field, vol);
+    smearing_APE_temporal(1, 0.3, pu);
+    write_field(out, "ape_temporal.bin", field, vol);
+
+    memcpy(field, orig, sizeof(double) * 16 * vol);
+    gaugefieldUpdate(1, 2, 1, 0);
+    write_field(out, "update.bin", field, vol);
+    project_gfield_to_sun(pu);
+    write_field(out, "project.bin", field, vol);
+    fprintf(s, "plaq_after %.17g\n", plaquette());
+    fclose(s);
+    lgt_compat_release();
+    return 0;
+}

@@ -26,6 +26,28 @@ def test_engine_exports_header_symbols():
     assert not missing, missing
 
 
+def test_compat_exports_reference_names():
+    """liblgt_compat.so defines the reference's entry points; the globals and
+    services it needs (pu, runParams, logging, ...) stay undefined, to be
+    resolved against the host program."""
+    import subprocess
+    path = os.path.join(ROOT, "masterarbeit_amd", "liblgt_compat.so")
+    if not os.path.exists(path):
+        pytest.fail("liblgt_compat.so not built (run __graft_entry__.build())")
+    text = open(os.path.join(ROOT, "include", "lgt_compat.h")).read()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    exported = set(re.findall(r"^(?:void|double)\s+(\w+)\s*\(", text, flags=re.M))
+    assert {"gaugefieldUpdate", "plaquette", "measureWilsonLoop", "topologicalCharge",
+            "smearing_APE_spatial", "meas_topologicalcharge"} <= exported
+    nm = subprocess.run(["nm", "-D", path], capture_output=True, text=True, check=True).stdout
+    defined = {l.split()[-1] for l in nm.splitlines() if " T " in l}
+    # error() also exists in glibc; the reference's definition in the program
+    # comes first in the lookup order and is the one bound at run time
+    undefined = {l.split()[-1].split("@")[0] for l in nm.splitlines() if " U " in l}
+    assert exported <= defined
+    assert {"pu", "runParams", "latParams", "measParams", "logging", "error", "getTime", "ranlxd"} <= undefined
+
+
 def test_python_binding_covers_header():
     from masterarbeit_amd.engine import SIGNATURES
     assert set(declared("lgt_su2.h")) == set(SIGNATURES)
