@@ -782,3 +782,174 @@ double or_qtop_clover(const olat_t *lat, const double *U, int seq, double *qx, d
     if (abs_sum) *abs_sum = nsum_val(&b) / (4. * OR_PI * OR_PI);
     return q / (4. * OR_PI * OR_PI);
 }
+
+/* ------------------------------------------------------------------------ */
+/* adjoint of spatial APE smearing (no reference counterpart)               */
+/*                                                                          */
+/* Gradients are Euclidean in the 4 real components of every link:         */
+/* <a,b> = a0 b0 + a.b = Re tr(a b^+)/2.  Right multiplication by q has      */
+/* adjoint right multiplication by q^+, left multiplication likewise, so     */
+/* for P = A B C with upstream H: dA = H (B C)^+, dB = A^+ H C^+,            */
+/* dC = (A B)^+ H.  This file scatters every staple term's three factor      */
+/* gradients (the engine gathers them per link instead).                    */
+/* ------------------------------------------------------------------------ */
+static void qsub_scaled(double *a, const double *u, double s)
+{
+    for (int c = 0; c < 4; c++) a[c] -= s * u[c];
+}
+
+static double qdot4(const double *a, const double *b)
+{
+    return a[0] * b[0] + a[1] * b[1] + a[2] * b[2] + a[3] * b[3];
+}
+
+static void qacc(double *a, const double *b, double s)
+{
+    for (int c = 0; c < 4; c++) a[c] += s * b[c];
+}
+
+/* triple products with daggers: flags bit0/1/2 dagger the 1st/2nd/3rd factor */
+static void qmul3(const double *a, const double *b, const double *c, int dag, double *o)
+{
+    double A[4], B[4], Cc[4], t[4];
+    memcpy(A, a, sizeof(A)); memcpy(B, b, sizeof(B)); memcpy(Cc, c, sizeof(Cc));
+    if (dag & 1) qdag(a, A);
+    if (dag & 2) qdag(b, B);
+    if (dag & 4) qdag(c, Cc);
+    or_qmul(A, B, t);
+    or_qmul(t, Cc, o);
+}
+
+/* one spatial APE step U -> U' storing |M| = |U + sp S| per spatial link */
+static void ape_spatial_step_save(const olat_t *lat, const double *Z, double *Uo, double *Mn, double sp)
+{
+    memcpy(Uo, Z, sizeof(double) * 16 * (size_t)lat->vol);
+    for (int64_t n = 0; n < lat->vol; n++)
+        for (int j = 1; j < 4; j++) {
+            double S[4], M[4];
+            or_ape_staple(lat, Z, n, j, 0, S);
+            for (int c = 0; c < 4; c++) M[c] = CLINK(Z, n, j)[c] + sp * S[c];
+            double m = sqrt(qdot4(M, M));
+            Mn[4 * n + j] = m;
+            for (int c = 0; c < 4; c++) LINK(Uo, n, j)[c] = M[c] / m;
+        }
+}
+
+/* backward of one step: Gout = dL/dU' -> Gin = dL/dZ */
+static void ape_spatial_step_backward(const olat_t *lat, const double *Z, const double *Uo, const double *Mn,
+                                      double sp, const double *Gout, double *Gin)
+{
+    const int64_t V = lat->vol;
+    double *H = (double *)calloc((size_t)16 * V, sizeof(double));
+    memcpy(Gin, Gout, sizeof(double) * 16 * (size_t)V);     /* temporal links pass through */
+    for (int64_t n = 0; n < V; n++)
+        for (int j = 1; j < 4; j++) {
+            double g[4];
+            memcpy(g, CLINK(Gout, n, j), sizeof(g));
+            qsub_scaled(g, CLINK(Uo, n, j), qdot4(g, CLINK(Uo, n, j)));
+            for (int c = 0; c < 4; c++) {
+                g[c] /= Mn[4 * n + j];
+                H[16 * n + 4 * j + c] = g[c];
+                LINK(Gin, n, j)[c] = g[c];                    /* dM/dZ = 1 */
+            }
+        }
+    for (int64_t x = 0; x < V; x++)
+        for (int j = 1; j < 4; j++) {
+            double h[4], t[4];
+            for (int c = 0; c < 4; c++) h[c] = sp * H[16 * x + 4 * j + c];
+            for (int k = 1; k < 4; k++) {
+                if (k == j) continue;
+                int64_t xk = NB(lat, x, k), xj = NB(lat, x, j);
+                /* forward term Z_k(x) Z_j(x+k) Z_k(x+j)^+ */
+                qmul3(h, CLINK(Z, xj, k), CLINK(Z, xk, j), 4, t);          /* h Z_k(x+j) Z_j(x+k)^+ */
+                qacc(LINK(Gin, x, k), t, 1.0);
+                qmul3(CLINK(Z, x, k), h, CLINK(Z, xj, k), 1, t);          /* Z_k(x)^+ h Z_k(x+j) */
+                qacc(LINK(Gin, xk, j), t, 1.0);
+                qmul3(h, CLINK(Z, x, k), CLINK(Z, xk, j), 1, t);          /* h^+ Z_k(x) Z_j(x+k) */
+                qacc(LINK(Gin, xj, k), t, 1.0);
+                /* backward term Z_k(x-k)^+ Z_j(x-k) Z_k(x-k+j) */
+                int64_t xm = NB(lat, x, k + 4), xmj = NB(lat, xm, j);
+                qmul3(CLINK(Z, xm, j), CLINK(Z, xmj, k), h, 4, t);        /* Z_j(x-k) Z_k(x-k+j) h^+ */
+                qacc(LINK(Gin, xm, k), t, 1.0);
+                qmul3(CLINK(Z, xm, k), h, CLINK(Z, xmj, k), 4, t);        /* Z_k(x-k) h Z_k(x-k+j)^+ */
+                qacc(LINK(Gin, xm, j), t, 1.0);
+                qmul3(CLINK(Z, xm, j), CLINK(Z, xm, k), h, 1, t);         /* Z_j(x-k)^+ Z_k(x-k) h */
+                qacc(LINK(Gin, xmj, k), t, 1.0);
+            }
+        }
+    free(H);
+}
+
+/* the R x T loops of measureWilsonLoop summed without normalisation:
+ * L = sum_x sum_{d=1..3} 2 [W(x; 0, d, T, R)]_0  (wilson.c:21-56) */
+double or_wloop_objective(const olat_t *lat, const double *U, int R, int T)
+{
+    nsum_t a = { 0, 0 };
+    for (int64_t n = 0; n < lat->vol; n++)
+        for (int d = 1; d < 4; d++) nsum_add(&a, or_wloop_site(lat, U, n, 0, d, T, R));
+    return nsum_val(&a);
+}
+
+/* dL/dU of the loop sum: for the link A at position k of a loop P = X A Y,
+ * d(2 Re[P]_0)/dA = 2 (Y X)^+, and for a daggered link 2 Y X. */
+static void wloop_grad(const olat_t *lat, const double *U, int R, int T, double *G)
+{
+    int m = 2 * (R + T);
+    int64_t *site = (int64_t *)malloc(sizeof(int64_t) * m);
+    int *dir = (int *)malloc(sizeof(int) * m), *dag = (int *)malloc(sizeof(int) * m);
+    double *pre = (double *)malloc(sizeof(double) * 4 * (m + 1)), *suf = (double *)malloc(sizeof(double) * 4 * (m + 1));
+    memset(G, 0, sizeof(double) * 16 * (size_t)lat->vol);
+    for (int64_t x = 0; x < lat->vol; x++)
+        for (int d = 1; d < 4; d++) {
+            int64_t nn = x;
+            int k = 0;
+            for (int i = 0; i < T; i++) { site[k] = nn; dir[k] = 0; dag[k++] = 0; nn = NB(lat, nn, 0); }
+            for (int i = 0; i < R; i++) { site[k] = nn; dir[k] = d; dag[k++] = 0; nn = NB(lat, nn, d); }
+            for (int i = 0; i < T; i++) { nn = NB(lat, nn, 4); site[k] = nn; dir[k] = 0; dag[k++] = 1; }
+            for (int i = 0; i < R; i++) { nn = NB(lat, nn, d + 4); site[k] = nn; dir[k] = d; dag[k++] = 1; }
+            qunit(pre);
+            for (int i = 0; i < m; i++) {
+                double A[4];
+                if (dag[i]) qdag(CLINK(U, site[i], dir[i]), A);
+                else memcpy(A, CLINK(U, site[i], dir[i]), sizeof(A));
+                or_qmul(pre + 4 * i, A, pre + 4 * (i + 1));
+            }
+            qunit(suf + 4 * m);
+            for (int i = m - 1; i >= 0; i--) {
+                double A[4];
+                if (dag[i]) qdag(CLINK(U, site[i], dir[i]), A);
+                else memcpy(A, CLINK(U, site[i], dir[i]), sizeof(A));
+                or_qmul(A, suf + 4 * (i + 1), suf + 4 * i);
+            }
+            for (int i = 0; i < m; i++) {
+                double YX[4], g[4];
+                or_qmul(suf + 4 * (i + 1), pre + 4 * i, YX);
+                if (dag[i]) memcpy(g, YX, sizeof(g));
+                else qdag(YX, g);
+                qacc(LINK(G, site[i], dir[i]), g, 2.0);
+            }
+        }
+    free(site); free(dir); free(dag); free(pre); free(suf);
+}
+
+double or_ape_vjp(const olat_t *lat, const double *U, int nsteps, double sp, int R, int T, double *G)
+{
+    size_t fb = sizeof(double) * 16 * (size_t)lat->vol;
+    double *fields = (double *)malloc(fb * (size_t)(nsteps + 1));
+    double *mn = (double *)malloc(sizeof(double) * 4 * (size_t)lat->vol * (nsteps > 0 ? nsteps : 1));
+    double *g2 = (double *)malloc(fb);
+    memcpy(fields, U, fb);
+    for (int s = 0; s < nsteps; s++)
+        ape_spatial_step_save(lat, fields + 16 * lat->vol * s, fields + 16 * lat->vol * (s + 1),
+                              mn + 4 * lat->vol * s, sp);
+    const double *Uf = fields + 16 * lat->vol * nsteps;
+    double L = or_wloop_objective(lat, Uf, R, T);
+    wloop_grad(lat, Uf, R, T, G);
+    for (int s = nsteps - 1; s >= 0; s--) {
+        ape_spatial_step_backward(lat, fields + 16 * lat->vol * s, fields + 16 * lat->vol * (s + 1),
+                                  mn + 4 * lat->vol * s, sp, G, g2);
+        memcpy(G, g2, fb);
+    }
+    free(fields); free(mn); free(g2);
+    return L;
+}

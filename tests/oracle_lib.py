@@ -77,6 +77,8 @@ class Oracle:
                 "or_wloop_site": (C.c_double, [P, _dp, C.c_int64, C.c_int, C.c_int, C.c_int, C.c_int]),
                 "or_qtop_improved": (C.c_double, [P, _dp, C.c_int, _dp, _dp]),
                 "or_qtop_clover": (C.c_double, [P, _dp, C.c_int, _dp, _dp]),
+                "or_wloop_objective": (C.c_double, [P, _dp, C.c_int, C.c_int]),
+                "or_ape_vjp": (C.c_double, [P, _dp, C.c_int, C.c_double, C.c_int, C.c_int, _dp]),
             }
             for name, (res, args) in sig.items():
                 f = getattr(lib, name)
@@ -167,6 +169,16 @@ class Oracle:
         f = self.lib().or_qtop_improved if kind == "improved" else self.lib().or_qtop_clover
         q = f(C.byref(self.lat), _ptr(U), 1 if seq else 0, _ptr(qx) if density else None, C.byref(s))
         return (q, s.value, qx) if density else (q, s.value)
+
+
+    # --- adjoint of spatial APE
+    def wloop_objective(self, U, R, T):
+        return self.lib().or_wloop_objective(C.byref(self.lat), _ptr(U), R, T)
+
+    def ape_vjp(self, U, nsteps, sp, R, T):
+        G = self.field()
+        L = self.lib().or_ape_vjp(C.byref(self.lat), _ptr(U), nsteps, sp, R, T, _ptr(G))
+        return L, G
 
 
 def philox(ctr, key):
