@@ -65,7 +65,10 @@ typedef struct lgt_engine lgt_engine;
 #define LGT_K_QTOP 6
 #define LGT_K_REDUCE 7
 #define LGT_K_FIELD 8      /* cold/hot start, layout conversion */
-#define LGT_K_COUNT 9
+#define LGT_K_APE_SAVE 9   /* smearing steps recomputed by the adjoint */
+#define LGT_K_VJP 10       /* adjoint steps of the smearing */
+#define LGT_K_WLOOP_GRAD 11
+#define LGT_K_COUNT 12
 
 /* ---- life cycle ---------------------------------------------------------- */
 
@@ -150,6 +153,31 @@ int lgt_wilson_loops(lgt_engine *e, int nloop, const int *R, const int *T, doubl
  * the same sum over |q(x)|, the scale of its rounding error; density
  * (nullable): q(x) per site in lexicographic order, VOL doubles. */
 int lgt_qtop(lgt_engine *e, int kind, double *q, double *abs_sum, double *density);
+
+/* ---- adjoint of spatial APE smearing (no reference counterpart) -----------
+ * A forward/backward pair for the vector-Jacobian product of N spatial APE
+ * steps (smearing_APE_spatial, LGT_Code/modules/obs/smearing.c:107-138).
+ *
+ * lgt_ape_vjp_forward smears the current field like lgt_ape(LGT_APE_SPATIAL,
+ * nsteps, sp) and keeps a checkpoint of the spatial links every `segment`
+ * steps (0: ceil(sqrt(nsteps))).  lgt_ape_vjp_backward then maps a cotangent
+ * dL/dU of the smeared field to dL/dU of the thin field, recomputing each
+ * segment's steps from its checkpoint (storing every step's |U + sp S|) and
+ * running the adjoint steps backwards through it.  Cotangents and gradients
+ * have 4 real components per link, Euclidean (<a,b> = a0 b0 + a.b), in the
+ * field layout of lgt_upload; the temporal links, untouched by spatial
+ * smearing, pass their cotangent through.  cotangent may be NULL to use the
+ * one left on the device by lgt_wloop_grad.  The smeared field stays current. */
+int lgt_ape_vjp_forward(lgt_engine *e, int nsteps, double sp, int segment);
+int lgt_ape_vjp_backward(lgt_engine *e, const double *cotangent, double *grad);
+/* The R x T loops of measureWilsonLoop (LGT_Code/modules/obs/wilson.c:21-56)
+ * summed without normalisation, L = sum_x sum_{d=1..3} 2 Re[W(x; t, d)]_0,
+ * returned as an unevaluated double-double sum[0] + sum[1]. */
+int lgt_wloop_sum(lgt_engine *e, int R, int T, double sum[2]);
+/* dL/dU of that L for every link of the current field; kept on the device as
+ * the pending cotangent of lgt_ape_vjp_backward, and copied to grad (field
+ * layout) unless grad is NULL. */
+int lgt_wloop_grad(lgt_engine *e, int R, int T, double *grad);
 
 /* ---- per-link harnesses (injected inputs, FP64) ---------------------------
  * The per-link heatbath of localHeatbathUpdate after the staple

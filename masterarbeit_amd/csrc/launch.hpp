@@ -12,6 +12,8 @@ namespace lgt {
 
 // raw plane pointers of one field: p[mu][parity]
 struct RawPlanes { void* p[4][2]; };
+// raw planes of one scalar per link (|M| of a smearing step)
+struct RawNorms { void* p[4][2]; };
 
 struct WGroupHost { int R, Tmax, slot_off, pad; };
 
@@ -37,6 +39,14 @@ template <typename T> struct Launch {
     // partial: 2 rows of 2*ceil(vh/256) columns (sum q, sum |q|)
     static hipError_t qtop(hipStream_t s, const Geo& g, const RawPlanes& P, int kind, void* partial,
                            double* density);
+    // adjoint of spatial APE (vjp_kernels.hpp)
+    static hipError_t ape_save(hipStream_t s, const Geo& g, const RawPlanes& in, const RawPlanes& out,
+                               const RawNorms& mn, double sp);
+    static hipError_t vjp_tangent(hipStream_t s, const Geo& g, const RawPlanes& U, const RawPlanes& G,
+                                  const RawNorms& mn, const RawPlanes& H);
+    static hipError_t vjp_step(hipStream_t s, const Geo& g, const RawPlanes& Z, const RawPlanes& Hin,
+                               const RawNorms* mprev, const RawPlanes& out, double sp);
+    static hipError_t wloop_grad(hipStream_t s, const Geo& g, const RawPlanes& U, int R, int Tt, const RawPlanes& G);
 };
 
 // precision-independent launchers (lgt_f64.hip)

@@ -3,6 +3,7 @@
 
 #include "kernels.hpp"
 #include "launch.hpp"
+#include "vjp_kernels.hpp"
 
 namespace lgt {
 
@@ -116,6 +117,50 @@ hipError_t Launch<T>::qtop(hipStream_t s, const Geo& g, const RawPlanes& P, int 
     dim3 grid(grid_x(g), 2), block(BLOCK);
     if (kind == 0) hipLaunchKernelGGL((k_qtop<T, 0>), grid, block, 0, s, planes<T>(P), g, (DD*)partial, density);
     else hipLaunchKernelGGL((k_qtop<T, 1>), grid, block, 0, s, planes<T>(P), g, (DD*)partial, density);
+    return hipGetLastError();
+}
+
+template <typename T>
+static NormPlanes<T> norms(const RawNorms& R)
+{
+    NormPlanes<T> N;
+    for (int mu = 0; mu < 4; mu++)
+        for (int p = 0; p < 2; p++) N.p[mu][p] = (T*)R.p[mu][p];
+    return N;
+}
+
+template <typename T>
+hipError_t Launch<T>::ape_save(hipStream_t s, const Geo& g, const RawPlanes& in, const RawPlanes& out,
+                               const RawNorms& mn, double sp)
+{
+    hipLaunchKernelGGL(k_ape_save<T>, dim3(grid_x(g), 6), dim3(BLOCK), 0, s, planes<T>(in), planes<T>(out), norms<T>(mn),
+                       g, (T)sp);
+    return hipGetLastError();
+}
+
+template <typename T>
+hipError_t Launch<T>::vjp_tangent(hipStream_t s, const Geo& g, const RawPlanes& U, const RawPlanes& G,
+                                  const RawNorms& mn, const RawPlanes& H)
+{
+    hipLaunchKernelGGL(k_vjp_tangent<T>, dim3(grid_x(g), 6), dim3(BLOCK), 0, s, planes<T>(U), planes<T>(G), norms<T>(mn),
+                       planes<T>(H), g);
+    return hipGetLastError();
+}
+
+template <typename T>
+hipError_t Launch<T>::vjp_step(hipStream_t s, const Geo& g, const RawPlanes& Z, const RawPlanes& Hin,
+                               const RawNorms* mprev, const RawPlanes& out, double sp)
+{
+    RawNorms none{};
+    hipLaunchKernelGGL(k_vjp_step<T>, dim3(grid_x(g), 6), dim3(BLOCK), 0, s, planes<T>(Z), planes<T>(Hin),
+                       norms<T>(mprev ? *mprev : none), mprev ? 1 : 0, planes<T>(out), g, (T)sp);
+    return hipGetLastError();
+}
+
+template <typename T>
+hipError_t Launch<T>::wloop_grad(hipStream_t s, const Geo& g, const RawPlanes& U, int R, int T_, const RawPlanes& G)
+{
+    hipLaunchKernelGGL(k_wloop_grad<T>, dim3(grid_x(g), 8), dim3(BLOCK), 0, s, planes<T>(U), g, R, T_, planes<T>(G));
     return hipGetLastError();
 }
 

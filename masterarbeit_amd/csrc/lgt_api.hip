@@ -47,6 +47,21 @@ struct ProfEvent {
     hipEvent_t a, b;
 };
 
+// device state of the APE adjoint: checkpoints, one segment of recomputed
+// steps with their |M|, two cotangent buffers and the objective's gradient
+struct VjpState {
+    int nsteps = 0, seg = 0, nck = 0;
+    double sp = 0.0;
+    bool fwd = false, cot = false;
+    std::vector<void*> ck, segU, segM;
+    std::vector<RawPlanes> ckP, segUP;
+    std::vector<RawNorms> segMN;
+    void* g[2] = {nullptr, nullptr};
+    RawPlanes gP[2];
+    void* gobj = nullptr;
+    RawPlanes gobjP;
+};
+
 }  // namespace
 
 struct lgt_engine {
@@ -75,6 +90,7 @@ struct lgt_engine {
     double prof_ms[LGT_K_COUNT] = {0};
     int64_t prof_n[LGT_K_COUNT] = {0};
     int64_t dev_bytes = 0;
+    VjpState vjp;
 };
 
 namespace {
@@ -176,8 +192,8 @@ int tile_order_ok(const Geo& g, int order)
     return order;
 }
 
-// reduces `nrows` rows of `ncols` double-doubles to host doubles (hi + lo)
-int reduce_to_host(lgt_engine* e, int nrows, int ncols, double* out)
+// reduces `nrows` rows of `ncols` double-doubles to host (hi, lo) pairs
+int reduce_to_host_dd(lgt_engine* e, int nrows, int ncols, double* out2)
 {
     TRY(ensure(e, &e->dres, &e->dres_bytes, sizeof(double) * 2 * (size_t)nrows, "results"));
     TRY(ensure_host(e, sizeof(double) * 2 * (size_t)nrows));
@@ -185,7 +201,79 @@ int reduce_to_host(lgt_engine* e, int nrows, int ncols, double* out)
     HIP_TRY(hipMemcpyAsync(e->hres, e->dres, sizeof(double) * 2 * nrows, hipMemcpyDeviceToHost, e->stream),
             "hipMemcpyAsync");
     HIP_TRY(hipStreamSynchronize(e->stream), "hipStreamSynchronize");
-    for (int r = 0; r < nrows; r++) out[r] = e->hres[2 * r] + e->hres[2 * r + 1];
+    for (int r = 0; r < 2 * nrows; r++) out2[r] = e->hres[r];
+    return LGT_OK;
+}
+
+// same, rounded to one double per row
+int reduce_to_host(lgt_engine* e, int nrows, int ncols, double* out)
+{
+    std::vector<double> dd(2 * (size_t)nrows);
+    TRY(reduce_to_host_dd(e, nrows, ncols, dd.data()));
+    for (int r = 0; r < nrows; r++) out[r] = dd[2 * r] + dd[2 * r + 1];
+    return LGT_OK;
+}
+
+int alloc_field(lgt_engine* e, void** base, RawPlanes* P)
+{
+    size_t bytes = (size_t)4 * e->vol * e->link_bytes, have = 0;
+    *base = nullptr;
+    TRY(ensure(e, base, &have, bytes, "adjoint field"));
+    set_planes(*P, *base, e->geo.vh, e->link_bytes);
+    return LGT_OK;
+}
+
+int alloc_norms(lgt_engine* e, void** base, RawNorms* N)
+{
+    size_t el = e->link_bytes / 4, bytes = (size_t)4 * e->vol * el, have = 0;
+    *base = nullptr;
+    TRY(ensure(e, base, &have, bytes, "adjoint norms"));
+    for (int mu = 0; mu < 4; mu++)
+        for (int p = 0; p < 2; p++) N->p[mu][p] = (char*)*base + ((size_t)(mu * 2 + p) * e->geo.vh) * el;
+    return LGT_OK;
+}
+
+void free_buf(lgt_engine* e, void*& b, size_t bytes)
+{
+    if (b) {
+        (void)hipFree(b);
+        e->dev_bytes -= (int64_t)bytes;
+    }
+    b = nullptr;
+}
+
+void free_vjp(lgt_engine* e, bool keep_cot)
+{
+    VjpState& v = e->vjp;
+    size_t fb = (size_t)4 * e->vol * e->link_bytes, nb = (size_t)4 * e->vol * (e->link_bytes / 4);
+    for (auto& b : v.ck) free_buf(e, b, fb);
+    for (auto& b : v.segU) free_buf(e, b, fb);
+    for (auto& b : v.segM) free_buf(e, b, nb);
+    v.ck.clear(); v.segU.clear(); v.segM.clear();
+    v.ckP.clear(); v.segUP.clear(); v.segMN.clear();
+    free_buf(e, v.g[0], fb);
+    free_buf(e, v.g[1], fb);
+    if (!keep_cot) {
+        free_buf(e, v.gobj, fb);
+        v.cot = false;
+    }
+    v.fwd = false;
+}
+
+int copy_spatial(lgt_engine* e, const RawPlanes& dst, const RawPlanes& src)
+{
+    size_t bytes = (size_t)e->geo.vh * e->link_bytes;
+    for (int mu = 1; mu < 4; mu++)
+        for (int p = 0; p < 2; p++)
+            HIP_TRY(hipMemcpyAsync(dst.p[mu][p], src.p[mu][p], bytes, hipMemcpyDeviceToDevice, e->stream),
+                    "hipMemcpyAsync D2D");
+    return LGT_OK;
+}
+
+int check_loop(const lgt_engine* e, int R, int T)
+{
+    if (T < 1 || T > e->L[0] - 1 || T > WMAXT_HOST || R < 1)
+        return fail(LGT_EINVAL, "loop R=%d T=%d: need R >= 1 and 1 <= T <= %d", R, T, e->L[0] - 1);
     return LGT_OK;
 }
 
@@ -277,6 +365,7 @@ int lgt_destroy(lgt_engine* e)
         hipEventDestroy(ev.a);
         hipEventDestroy(ev.b);
     }
+    free_vjp(e, false);
     if (e->field) hipFree(e->field);
     if (e->field_alt) hipFree(e->field_alt);
     if (e->partial) hipFree(e->partial);
@@ -548,6 +637,152 @@ int lgt_qtop(lgt_engine* e, int kind, double* q, double* abs_sum, double* densit
     if (density) {
         HIP_TRY(hipMemcpy(density, dq, sizeof(double) * (size_t)e->vol, hipMemcpyDeviceToHost), "hipMemcpy D2H");
     }
+    return LGT_OK;
+}
+
+int lgt_ape_vjp_forward(lgt_engine* e, int nsteps, double sp, int segment)
+{
+    if (!e || nsteps < 1 || segment < 0) return fail(LGT_EINVAL, "lgt_ape_vjp_forward: nsteps %d segment %d", nsteps, segment);
+    TRY(set_dev(e));
+    VjpState& v = e->vjp;
+    free_vjp(e, false);
+    int seg = segment > 0 ? segment : (int)std::ceil(std::sqrt((double)nsteps));
+    if (seg > nsteps) seg = nsteps;
+    v.nsteps = nsteps;
+    v.seg = seg;
+    v.nck = (nsteps + seg - 1) / seg;
+    v.sp = sp;
+    v.ck.assign(v.nck, nullptr);
+    v.ckP.resize(v.nck);
+    for (int c = 0; c < v.nck; c++) {
+        int rc = alloc_field(e, &v.ck[c], &v.ckP[c]);
+        if (rc != LGT_OK) {
+            free_vjp(e, false);
+            return rc;
+        }
+    }
+    if (!e->field_alt) {
+        size_t have = 0;
+        TRY(ensure(e, &e->field_alt, &have, (size_t)4 * e->vol * e->link_bytes, "APE buffer"));
+        set_planes(e->alt, e->field_alt, e->geo.vh, e->link_bytes);
+    }
+    for (int n = 0; n < nsteps; n++) {
+        if (n % seg == 0) TRY(copy_spatial(e, v.ckP[n / seg], e->cur));
+        TRY(run(e, LGT_K_APE, "ape", [&] { return DISPATCH(e, ape(e->stream, e->geo, e->cur, e->alt, 0, sp)); }));
+        for (int mu = 1; mu < 4; mu++)
+            for (int p = 0; p < 2; p++) std::swap(e->cur.p[mu][p], e->alt.p[mu][p]);
+    }
+    v.fwd = true;
+    HIP_TRY(hipStreamSynchronize(e->stream), "hipStreamSynchronize");
+    return LGT_OK;
+}
+
+int lgt_wloop_sum(lgt_engine* e, int R, int T, double sum[2])
+{
+    if (!e || !sum) return fail(LGT_EINVAL, "lgt_wloop_sum: null argument");
+    TRY(check_loop(e, R, T));
+    TRY(set_dev(e));
+    const int ncols = 6 * (int)nbx(e);
+    std::vector<int> slots(T, -1);
+    slots[T - 1] = 0;
+    WGroupHost gh{R, T, 0, 0};
+    size_t part_bytes = sizeof(double) * 2 * (size_t)ncols;
+    size_t off = (part_bytes + 255) & ~(size_t)255;
+    TRY(ensure(e, &e->partial, &e->partial_bytes, off + sizeof(gh) + sizeof(int) * T, "partials"));
+    WGroupHost* dg = (WGroupHost*)((char*)e->partial + off);
+    int* ds = (int*)((char*)e->partial + off + sizeof(gh));
+    HIP_TRY(hipMemcpyAsync(dg, &gh, sizeof(gh), hipMemcpyHostToDevice, e->stream), "hipMemcpyAsync");
+    HIP_TRY(hipMemcpyAsync(ds, slots.data(), sizeof(int) * T, hipMemcpyHostToDevice, e->stream), "hipMemcpyAsync");
+    TRY(run(e, LGT_K_WLOOP, "wilson loops",
+            [&] { return DISPATCH(e, wloops(e->stream, e->geo, e->cur, dg, 1, ds, e->partial, ncols)); }));
+    return reduce_to_host_dd(e, 1, ncols, sum);
+}
+
+static int field_to_host(lgt_engine* e, const RawPlanes& P, double* aos)
+{
+    size_t bytes = sizeof(double) * 16 * (size_t)e->vol;
+    TRY(ensure(e, &e->staging, &e->staging_bytes, bytes, "staging"));
+    TRY(run(e, LGT_K_FIELD, "to_aos", [&] { return DISPATCH(e, to_aos(e->stream, e->geo, P, (double*)e->staging)); }));
+    HIP_TRY(hipMemcpyAsync(aos, e->staging, bytes, hipMemcpyDeviceToHost, e->stream), "hipMemcpyAsync D2H");
+    HIP_TRY(hipStreamSynchronize(e->stream), "hipStreamSynchronize");
+    return LGT_OK;
+}
+
+int lgt_wloop_grad(lgt_engine* e, int R, int T, double* grad)
+{
+    if (!e) return fail(LGT_EINVAL, "lgt_wloop_grad: null engine");
+    TRY(check_loop(e, R, T));
+    TRY(set_dev(e));
+    VjpState& v = e->vjp;
+    if (!v.gobj) TRY(alloc_field(e, &v.gobj, &v.gobjP));
+    TRY(run(e, LGT_K_WLOOP_GRAD, "wilson-loop gradient",
+            [&] { return DISPATCH(e, wloop_grad(e->stream, e->geo, e->cur, R, T, v.gobjP)); }));
+    v.cot = true;
+    if (grad) return field_to_host(e, v.gobjP, grad);
+    HIP_TRY(hipStreamSynchronize(e->stream), "hipStreamSynchronize");
+    return LGT_OK;
+}
+
+int lgt_ape_vjp_backward(lgt_engine* e, const double* cotangent, double* grad)
+{
+    if (!e) return fail(LGT_EINVAL, "lgt_ape_vjp_backward: null engine");
+    VjpState& v = e->vjp;
+    if (!v.fwd) return fail(LGT_EINVAL, "lgt_ape_vjp_backward: no forward pass (call lgt_ape_vjp_forward)");
+    TRY(set_dev(e));
+    if (cotangent) {
+        if (!v.gobj) TRY(alloc_field(e, &v.gobj, &v.gobjP));
+        size_t bytes = sizeof(double) * 16 * (size_t)e->vol;
+        TRY(ensure(e, &e->staging, &e->staging_bytes, bytes, "staging"));
+        HIP_TRY(hipMemcpyAsync(e->staging, cotangent, bytes, hipMemcpyHostToDevice, e->stream), "hipMemcpyAsync H2D");
+        TRY(run(e, LGT_K_FIELD, "from_aos",
+                [&] { return DISPATCH(e, from_aos(e->stream, e->geo, v.gobjP, (const double*)e->staging)); }));
+        v.cot = true;
+    }
+    if (!v.cot) return fail(LGT_EINVAL, "lgt_ape_vjp_backward: no cotangent (pass one or call lgt_wloop_grad)");
+    if (v.segU.empty()) {
+        v.segU.assign(v.seg, nullptr);
+        v.segUP.resize(v.seg);
+        v.segM.assign(v.seg, nullptr);
+        v.segMN.resize(v.seg);
+        for (int k = 0; k < v.seg; k++) {
+            TRY(alloc_field(e, &v.segU[k], &v.segUP[k]));
+            TRY(alloc_norms(e, &v.segM[k], &v.segMN[k]));
+        }
+        TRY(alloc_field(e, &v.g[0], &v.gP[0]));
+        TRY(alloc_field(e, &v.g[1], &v.gP[1]));
+    }
+    int h = 0;   // gP[h]: the running cotangent
+    const double sp = v.sp;
+    for (int s = v.nck - 1; s >= 0; s--) {
+        const int n0 = s * v.seg, n1 = std::min(n0 + v.seg, v.nsteps);
+        for (int n = n0; n < n1; n++) {
+            const RawPlanes& in = (n == n0) ? v.ckP[s] : v.segUP[n - n0 - 1];
+            TRY(run(e, LGT_K_APE_SAVE, "ape recompute", [&] {
+                return DISPATCH(e, ape_save(e->stream, e->geo, in, v.segUP[n - n0], v.segMN[n - n0], sp));
+            }));
+        }
+        const RawPlanes& Utop = (s == v.nck - 1) ? e->cur : v.ckP[s + 1];
+        const RawPlanes& Gtop = (s == v.nck - 1) ? v.gobjP : v.gP[h];
+        const int o = (s == v.nck - 1) ? h : 1 - h;
+        TRY(run(e, LGT_K_VJP, "adjoint tangent", [&] {
+            return DISPATCH(e, vjp_tangent(e->stream, e->geo, Utop, Gtop, v.segMN[n1 - 1 - n0], v.gP[o]));
+        }));
+        h = o;
+        for (int n = n1 - 1; n >= n0; n--) {
+            const RawPlanes& Z = (n == n0) ? v.ckP[s] : v.segUP[n - n0 - 1];
+            const RawNorms* mprev = (n > n0) ? &v.segMN[n - 1 - n0] : nullptr;
+            TRY(run(e, LGT_K_VJP, "adjoint step", [&] {
+                return DISPATCH(e, vjp_step(e->stream, e->geo, Z, v.gP[h], mprev, v.gP[1 - h], sp));
+            }));
+            h = 1 - h;
+        }
+    }
+    // spatial links: the adjoint's result; temporal links: the cotangent itself
+    RawPlanes out = v.gP[h];
+    out.p[0][0] = v.gobjP.p[0][0];
+    out.p[0][1] = v.gobjP.p[0][1];
+    if (grad) return field_to_host(e, out, grad);
+    HIP_TRY(hipStreamSynchronize(e->stream), "hipStreamSynchronize");
     return LGT_OK;
 }
 

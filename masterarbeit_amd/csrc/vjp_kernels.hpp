@@ -1,0 +1,192 @@
+// vjp_kernels.hpp -- adjoint (vector-Jacobian product) of N-step spatial APE
+// smearing and the gradient of the Wilson-loop objective
+//   L = sum_x sum_{d=1..3} 2 Re[W_{R x T}(x; t, d)]_0 .
+// No reference counterpart.  Gradients are Euclidean in the 4 real link
+// components, <a,b> = a0 b0 + a.b = Re tr(a b^+)/2; right (left)
+// multiplication by q has adjoint right (left) multiplication by q^+.
+//
+// One forward step (Jacobi, spatial links a = 1..3):
+//   M_a(y) = Z_a(y) + sp S_a(y),   U'_a(y) = M_a(y) / |M_a(y)|
+// Backward of that step for a cotangent G' of U':
+//   H_a(y)  = (G'_a(y) - <G'_a(y), U'_a(y)> U'_a(y)) / |M_a(y)|      (tangent)
+//   G_a(y)  = H_a(y) + sp * sum_{b != a} (t1 + ... + t6)            (gather)
+// with, for every spatial b != a, the six places where Z_a(y) enters a staple:
+//   t1 = H_b(y) Z_a(y+b) Z_b(y+a)^+          t4 = Z_b(y) Z_a(y+b) H_b(y+a)^+
+//   t2 = Z_b(y-b)^+ H_a(y-b) Z_b(y-b+a)      t5 = Z_b(y) H_a(y+b) Z_b(y+a)^+
+//   t3 = H_b(y-b)^+ Z_a(y-b) Z_b(y-b+a)      t6 = Z_b(y-b)^+ Z_a(y-b) H_b(y-b+a)
+// Temporal links are not used by spatial staples: their cotangent passes
+// through every step unchanged.
+#pragma once
+
+#include "kernels.hpp"
+
+namespace lgt {
+
+// scalar planes of |M| (one value per spatial link; direction 0 unused)
+template <typename T> struct NormPlanes { T* p[4][2]; };
+
+template <typename T>
+__device__ __forceinline__ T nload(const NormPlanes<T>& N, const Geo& g, int mu, const Site& s)
+{
+    return N.p[mu][parity(s)][cb_index(g, s)];
+}
+
+// forward spatial APE step that also records |M| (values equal k_ape's)
+template <typename T>
+__global__ void __launch_bounds__(BLOCK) k_ape_save(Planes<T> Z, Planes<T> O, NormPlanes<T> Mn, Geo g, T sp)
+{
+    uint32_t i = tile_base(g) + threadIdx.x;
+    if (i >= g.vh) return;
+    int p = blockIdx.y & 1;
+    int mu = 1 + (blockIdx.y >> 1);
+    Site s = cb_site(g, i, p);
+    Q<T> S = qscale(ape_staple(Z, g, s, mu, 0), sp);
+    Q<T> M = qadd(qload<T>(Z.p[mu][p], i), S);
+    T m = qdet_sqrt(M);
+    Mn.p[mu][p][i] = m;
+    qstore<T>(O.p[mu][p], i, qproject(M));
+}
+
+// H = (G - <G,U> U) / |M| on the spatial links
+template <typename T>
+__global__ void __launch_bounds__(BLOCK) k_vjp_tangent(Planes<T> U, Planes<T> G, NormPlanes<T> Mn, Planes<T> H, Geo g)
+{
+#pragma clang fp contract(fast)
+    uint32_t i = blockIdx.x * BLOCK + threadIdx.x;
+    if (i >= g.vh) return;
+    int p = blockIdx.y & 1;
+    int mu = 1 + (blockIdx.y >> 1);
+    Q<T> u = qload<T>(U.p[mu][p], i), gr = qload<T>(G.p[mu][p], i);
+    T c = qdot(gr, u), m = Mn.p[mu][p][i];
+    Q<T> h{(gr.c0 - c * u.c0) / m, (gr.c1 - c * u.c1) / m, (gr.c2 - c * u.c2) / m, (gr.c3 - c * u.c3) / m};
+    qstore<T>(H.p[mu][p], i, h);
+}
+
+template <typename T>
+__device__ __forceinline__ Q<T> mul3(const Q<T>& a, const Q<T>& b, const Q<T>& c)
+{
+    return qmul(qmul(a, b), c);
+}
+
+// One backward step.  Z = input field of the step, Hin = tangent cotangent of
+// its output (H above).  Writes G = dL/dZ for the spatial links, or, when
+// Mprev is given (|M| of the step before), directly the tangent cotangent of
+// that earlier step: (G - <G,Z> Z)/Mprev  (Z is that step's output).
+template <typename T>
+__global__ void __launch_bounds__(BLOCK) k_vjp_step(Planes<T> Z, Planes<T> Hin, NormPlanes<T> Mprev, int has_prev,
+                                                    Planes<T> Out, Geo g, T sp)
+{
+#pragma clang fp contract(fast)
+    uint32_t i = tile_base(g) + threadIdx.x;
+    if (i >= g.vh) return;
+    int p = blockIdx.y & 1;
+    int a = 1 + (blockIdx.y >> 1);
+    Site y = cb_site(g, i, p);
+    Site ya = shift(g, y, a, 1);
+    Q<T> acc = qzero<T>();
+#pragma unroll
+    for (int b = 1; b < 4; b++) {
+        if (b == a) continue;
+        Site yb = shift(g, y, b, 1), ymb = shift(g, y, b, -1);
+        Site ymba = shift(g, ymb, a, 1);
+        Q<T> Za_yb = link(Z, g, a, yb), Zb_ya = link(Z, g, b, ya), Zb_y = link(Z, g, b, y);
+        Q<T> Zb_ymb = link(Z, g, b, ymb), Za_ymb = link(Z, g, a, ymb), Zb_ymba = link(Z, g, b, ymba);
+        Q<T> Hb_y = link(Hin, g, b, y), Ha_ymb = link(Hin, g, a, ymb), Hb_ymb = link(Hin, g, b, ymb);
+        Q<T> Hb_ya = link(Hin, g, b, ya), Ha_yb = link(Hin, g, a, yb), Hb_ymba = link(Hin, g, b, ymba);
+        acc = qadd(acc, mul3(Hb_y, Za_yb, qdag(Zb_ya)));            // t1
+        acc = qadd(acc, mul3(qdag(Zb_ymb), Ha_ymb, Zb_ymba));       // t2
+        acc = qadd(acc, mul3(qdag(Hb_ymb), Za_ymb, Zb_ymba));       // t3
+        acc = qadd(acc, mul3(Zb_y, Za_yb, qdag(Hb_ya)));            // t4
+        acc = qadd(acc, mul3(Zb_y, Ha_yb, qdag(Zb_ya)));            // t5
+        acc = qadd(acc, mul3(qdag(Zb_ymb), Za_ymb, Hb_ymba));       // t6
+    }
+    Q<T> G = qadd(qload<T>(Hin.p[a][p], i), qscale(acc, sp));
+    if (has_prev) {
+        Q<T> z = qload<T>(Z.p[a][p], i);
+        T c = qdot(G, z), m = Mprev.p[a][p][i];
+        G = Q<T>{(G.c0 - c * z.c0) / m, (G.c1 - c * z.c1) / m, (G.c2 - c * z.c2) / m, (G.c3 - c * z.c3) / m};
+    }
+    qstore<T>(Out.p[a][p], i, G);
+}
+
+// ---------------------------------------------------------------------------
+// gradient of L for every link, gathered per link over the loops that hold it
+// (the loop path: T steps +t, R steps +d, T steps -t, R steps -d from x).
+// For a link A at some position of a loop P = X A Y, d(2 Re[P]_0)/dA is
+// 2 (Y X)^+, and 2 Y X where the loop runs the link backwards; Y X is the
+// open path from the end of the link around the loop to its start.
+// ---------------------------------------------------------------------------
+template <typename T>
+__device__ __forceinline__ void seg(const Planes<T>& U, const Geo& g, Q<T>& acc, Site& s, int dir, int n, bool back)
+{
+    for (int k = 0; k < n; k++) {
+        if (!back) {
+            acc = qmul(acc, link(U, g, dir, s));
+            s = shift(g, s, dir, 1);
+        } else {
+            s = shift(g, s, dir, -1);
+            acc = qmul_dag(acc, link(U, g, dir, s));
+        }
+    }
+}
+
+// grid: x over cb sites, y = mu*2 + parity
+template <typename T>
+__global__ void __launch_bounds__(BLOCK) k_wloop_grad(Planes<T> U, Geo g, int R, int Tt, Planes<T> G)
+{
+#pragma clang fp contract(fast)
+    uint32_t i = tile_base(g) + threadIdx.x;
+    if (i >= g.vh) return;
+    int mu = blockIdx.y >> 1, p = blockIdx.y & 1;
+    Site y = cb_site(g, i, p);
+    Q<T> grad = qzero<T>();
+    if (mu == 0) {
+        for (int d = 1; d < 4; d++) {
+            for (int k = 0; k < Tt; k++) {
+                // left edge, position k: forward link
+                Q<T> acc = qunit<T>();
+                Site s = shift(g, y, 0, 1);
+                seg(U, g, acc, s, 0, Tt - 1 - k, false);
+                seg(U, g, acc, s, d, R, false);
+                seg(U, g, acc, s, 0, Tt, true);
+                seg(U, g, acc, s, d, R, true);
+                seg(U, g, acc, s, 0, k, false);
+                grad = qadd(grad, qdag(acc));
+                // right edge, position k: the loop runs the link backwards
+                acc = qunit<T>();
+                s = y;
+                seg(U, g, acc, s, 0, Tt - 1 - k, true);
+                seg(U, g, acc, s, d, R, true);
+                seg(U, g, acc, s, 0, Tt, false);
+                seg(U, g, acc, s, d, R, false);
+                seg(U, g, acc, s, 0, k, true);
+                grad = qadd(grad, acc);
+            }
+        }
+    } else {
+        const int d = mu;
+        for (int k = 0; k < R; k++) {
+            // top edge, position k: forward link
+            Q<T> acc = qunit<T>();
+            Site s = shift(g, y, d, 1);
+            seg(U, g, acc, s, d, R - 1 - k, false);
+            seg(U, g, acc, s, 0, Tt, true);
+            seg(U, g, acc, s, d, R, true);
+            seg(U, g, acc, s, 0, Tt, false);
+            seg(U, g, acc, s, d, k, false);
+            grad = qadd(grad, qdag(acc));
+            // bottom edge, position k: backwards
+            acc = qunit<T>();
+            s = y;
+            seg(U, g, acc, s, d, R - 1 - k, true);
+            seg(U, g, acc, s, 0, Tt, false);
+            seg(U, g, acc, s, d, R, false);
+            seg(U, g, acc, s, 0, Tt, true);
+            seg(U, g, acc, s, d, k, true);
+            grad = qadd(grad, acc);
+        }
+    }
+    qstore<T>(G.p[mu][p], i, qscale(grad, T(2)));
+}
+
+}  // namespace lgt

@@ -20,8 +20,10 @@ FP64, FP32 = 0, 1
 APE_SPATIAL, APE_ALL, APE_TEMPORAL = 0, 1, 2
 Q_IMPROVED, Q_CLOVER = 0, 1
 OPT_TILE_ORDER, OPT_PROFILE = 0, 1
-K_HEATBATH, K_OVERRELAX, K_PROJECT, K_PLAQUETTE, K_APE, K_WLOOP, K_QTOP, K_REDUCE, K_FIELD = range(9)
-KERNEL_CLASSES = ["heatbath", "overrelax", "project", "plaquette", "ape", "wloop", "qtop", "reduce", "field"]
+(K_HEATBATH, K_OVERRELAX, K_PROJECT, K_PLAQUETTE, K_APE, K_WLOOP, K_QTOP, K_REDUCE, K_FIELD, K_APE_SAVE, K_VJP,
+ K_WLOOP_GRAD) = range(12)
+KERNEL_CLASSES = ["heatbath", "overrelax", "project", "plaquette", "ape", "wloop", "qtop", "reduce", "field",
+                  "ape_save", "vjp", "wloop_grad"]
 
 _dp = C.POINTER(C.c_double)
 _ip = C.POINTER(C.c_int)
@@ -49,6 +51,10 @@ SIGNATURES = {
     "lgt_ape": (C.c_int, [C.c_void_p, C.c_int, C.c_int, C.c_double]),
     "lgt_wilson_loops": (C.c_int, [C.c_void_p, C.c_int, _ip, _ip, _dp]),
     "lgt_qtop": (C.c_int, [C.c_void_p, C.c_int, _dp, _dp, _dp]),
+    "lgt_ape_vjp_forward": (C.c_int, [C.c_void_p, C.c_int, C.c_double, C.c_int]),
+    "lgt_ape_vjp_backward": (C.c_int, [C.c_void_p, _dp, _dp]),
+    "lgt_wloop_sum": (C.c_int, [C.c_void_p, C.c_int, C.c_int, _dp]),
+    "lgt_wloop_grad": (C.c_int, [C.c_void_p, C.c_int, C.c_int, _dp]),
     "lgt_heatbath_harness": (C.c_int, [C.c_int, _dp, C.c_double, _dp, _ip, _ip, _dp, _ip]),
     "lgt_overrelax_harness": (C.c_int, [C.c_int, _dp, _dp, _dp]),
     "lgt_profile_read": (C.c_int, [C.c_void_p, C.c_int, _dp, C.POINTER(C.c_int64)]),
@@ -210,6 +216,29 @@ class Engine:
         _check(self._lib.lgt_qtop(self._h, kind, C.byref(q), C.byref(s), _ptr(qx) if density else None),
                "lgt_qtop")
         return (q.value, s.value, qx) if density else (q.value, s.value)
+
+
+    # ---- adjoint of spatial APE
+    def ape_vjp_forward(self, nsteps, sp, segment=0):
+        _check(self._lib.lgt_ape_vjp_forward(self._h, nsteps, sp, segment), "lgt_ape_vjp_forward")
+
+    def ape_vjp_backward(self, cotangent=None, grad=True):
+        cot = None if cotangent is None else np.ascontiguousarray(cotangent, dtype=np.float64)
+        G = np.empty((self.vol, 4, 4), dtype=np.float64) if grad else None
+        _check(self._lib.lgt_ape_vjp_backward(self._h, _ptr(cot) if cot is not None else None,
+                                              _ptr(G) if grad else None), "lgt_ape_vjp_backward")
+        return G
+
+    def wloop_sum(self, R, T):
+        """(hi, lo) of the unnormalised loop sum."""
+        out = np.zeros(2)
+        _check(self._lib.lgt_wloop_sum(self._h, R, T, _ptr(out)), "lgt_wloop_sum")
+        return out[0], out[1]
+
+    def wloop_grad(self, R, T, host=True):
+        G = np.empty((self.vol, 4, 4), dtype=np.float64) if host else None
+        _check(self._lib.lgt_wloop_grad(self._h, R, T, _ptr(G) if host else None), "lgt_wloop_grad")
+        return G
 
 
 def heatbath_harness(staples, beta, uniforms, offsets, lens):

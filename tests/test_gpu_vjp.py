@@ -1,0 +1,118 @@
+"""Adjoint of N-step spatial APE smearing with the Wilson-loop objective
+L = sum_x sum_d 2 Re[W_{RxT}]_0, through the C ABI.
+
+Checked against the FP64 CPU chain rule (oracle/su2_oracle.c, or_ape_vjp,
+which scatters each staple term's factor gradients while the engine gathers
+them per link), against central finite differences along su(2) directions
+computed by the engine itself (double-double loop sums), and across
+checkpoint segmentations (sqrt(N) default, uneven segments, one segment)."""
+import numpy as np
+import pytest
+
+import fields as F
+from oracle_lib import Oracle
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def lgt():
+    import masterarbeit_amd as m
+    m.load_library()
+    return m
+
+
+def su2_exp(X, eps):
+    n = np.linalg.norm(X[1:])
+    return np.concatenate([[np.cos(eps * n)], np.sin(eps * n) * X[1:] / n])
+
+
+def thermal(L, seed=3):
+    o = Oracle(L)
+    U = o.hot(seed)
+    for s in range(2):
+        o.engine_sweep(U, 3.09, 1, seed, s)
+    return o, U
+
+
+@pytest.mark.parametrize("L", [(6, 4, 4, 8), (8, 8, 8, 8)])
+def test_wloop_sum_and_gradient(lgt, L):
+    o, U = thermal(L)
+    R, T = 2, 3
+    with lgt.Engine(L, 0, 1) as e:
+        e.upload(U)
+        hi, lo = e.wloop_sum(R, T)
+        G = e.wloop_grad(R, T)
+    Lref, Gref = o.ape_vjp(U, 0, 0.25, R, T)
+    assert hi + lo == pytest.approx(Lref, rel=1e-14, abs=1e-12)
+    np.testing.assert_allclose(G, Gref, rtol=0, atol=1e-12 * np.abs(Gref).max())
+
+
+@pytest.mark.parametrize("L,nsteps,segment", [((6, 4, 4, 8), 7, 0), ((6, 4, 4, 8), 7, 3), ((8, 8, 8, 8), 5, 5),
+                                              ((8, 8, 8, 8), 9, 2)])
+def test_vjp_matches_cpu_chain_rule(lgt, L, nsteps, segment):
+    o, U = thermal(L)
+    R, T, sp = 2, 2, 0.25
+    Lref, Gref = o.ape_vjp(U, nsteps, sp, R, T)
+    with lgt.Engine(L, 0, 1) as e:
+        e.upload(U)
+        e.ape_vjp_forward(nsteps, sp, segment)
+        hi, lo = e.wloop_sum(R, T)
+        e.wloop_grad(R, T, host=False)
+        G = e.ape_vjp_backward()
+        V = e.download()
+    W = U.copy()
+    o.ape(W, 0, nsteps, sp)
+    np.testing.assert_array_equal(V, W)              # the forward pass is the plain smearing, bit for bit
+    assert hi + lo == pytest.approx(Lref, rel=1e-13)
+    scale = np.abs(Gref).max()
+    err = np.abs(G - Gref).max() / scale
+    print(f"VJP L={L} N={nsteps} seg={segment}: max rel err {err:.2e}")
+    assert err < 1e-11
+
+
+def test_vjp_explicit_cotangent(lgt):
+    L = (6, 4, 4, 8)
+    o, U = thermal(L)
+    rng = np.random.default_rng(4)
+    cot = rng.normal(size=(o.vol, 4, 4))
+    with lgt.Engine(L, 0, 1) as e:
+        e.upload(U)
+        e.ape_vjp_forward(4, 0.25)
+        G1 = e.ape_vjp_backward(cot)
+        G2 = e.ape_vjp_backward(cot)                 # repeatable, the forward state is kept
+    np.testing.assert_array_equal(G1, G2)
+    np.testing.assert_array_equal(G1[:, 0], cot[:, 0])   # temporal links pass through
+    # linearity in the cotangent
+    with lgt.Engine(L, 0, 1) as e:
+        e.upload(U)
+        e.ape_vjp_forward(4, 0.25)
+        G3 = e.ape_vjp_backward(2.5 * cot)
+    np.testing.assert_allclose(G3, 2.5 * G1, rtol=1e-13, atol=1e-13)
+
+
+@pytest.mark.parametrize("prec,rtol", [(0, 2e-6), (1, 2e-2)])
+def test_vjp_matches_gpu_finite_differences(lgt, prec, rtol):
+    L = (8, 8, 8, 8)
+    o, U = thermal(L)
+    R, T, sp, nsteps, eps = 3, 3, 0.25, 12, 1e-4
+    with lgt.Engine(L, prec, 1) as e:
+        e.upload(U)
+        e.ape_vjp_forward(nsteps, sp)
+        e.wloop_grad(R, T, host=False)
+        G = e.ape_vjp_backward()
+        rng = np.random.default_rng(9)
+        for _ in range(6):
+            n, mu = int(rng.integers(o.vol)), int(rng.integers(4))
+            X = np.concatenate([[0.0], rng.normal(size=3)])
+            vals = []
+            for sgn in (1, -1):
+                V = U.copy()
+                V[n, mu] = F.qmul(su2_exp(X, sgn * eps), U[n, mu])
+                e.upload(V)
+                e.ape(0, nsteps, sp)
+                hi, lo = e.wloop_sum(R, T)
+                vals.append((hi, lo))
+            fd = ((vals[0][0] - vals[1][0]) + (vals[0][1] - vals[1][1])) / (2 * eps)
+            an = float(np.dot(G[n, mu], F.qmul(X, U[n, mu])))
+            assert an == pytest.approx(fd, rel=rtol, abs=1e-7 if prec == 0 else 1e-3), (n, mu, an, fd)
