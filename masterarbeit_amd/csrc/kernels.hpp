@@ -487,20 +487,57 @@ __device__ __forceinline__ Q<T> ape_staple(const Planes<T>& Z, const Geo& g, con
     return acc;
 }
 
-// grid: x over cb sites, y = (mu - mu0)*2 + parity
+// One thread smears every link of its site that the mode smears, so the
+// gathers that several directions share (Z_k(n), Z_a(n+b), Z_a(n-b),
+// Z_a(n-a+b)) are loaded once: 24 loads for the 3 spatial links instead of
+// 36, 44 instead of 72 for all four.  Each staple is still summed in the
+// reference's order (k ascending, forward then backward term).
+// grid: x over cb sites, y = parity
 template <typename T, int MODE>
 __global__ void __launch_bounds__(BLOCK) k_ape(Planes<T> Z, Planes<T> O, Geo g, T sp)
 {
+    constexpr int MU0 = (MODE == 0) ? 1 : 0, MU1 = (MODE == 2) ? 0 : 3;
     uint32_t i = tile_base(g) + threadIdx.x;
     if (i >= g.vh) return;
-    int p = blockIdx.y & 1;
-    int mu = (MODE == 0 ? 1 : 0) + (blockIdx.y >> 1);
-    int dt = (MODE == 1) ? mu : 0;
+    const int p = blockIdx.y;
     Site s = cb_site(g, i, p);
-    Q<T> S = ape_staple(Z, g, s, mu, dt);
-    S = qscale(S, sp);
-    Q<T> U = qadd(qload<T>(Z.p[mu][p], i), S);
-    qstore<T>(O.p[mu][p], i, qproject(U));
+    // directions that enter the staples
+    constexpr int K0 = (MODE == 1) ? 0 : 1;
+    Q<T> z0[4], res[4];
+    Site sp_[4], sm_[4];
+#pragma unroll
+    for (int b = 0; b < 4; b++) {
+        sp_[b] = shift(g, s, b, 1);
+        sm_[b] = shift(g, s, b, -1);
+    }
+#pragma unroll
+    for (int a = 0; a < 4; a++) {
+        if (a < K0 && !(a >= MU0 && a <= MU1)) continue;
+        z0[a] = link(Z, g, a, s);
+    }
+#pragma unroll
+    for (int mu = MU0; mu <= MU1; mu++) {
+        Q<T> acc = qzero<T>();
+        const int dt = (MODE == 1) ? mu : 0;
+#pragma unroll
+        for (int k = 0; k < 4; k++) {
+            if (k == mu || k == dt) continue;
+            // forward: Z_k(n) Z_mu(n+k) Z_k(n+mu)^+
+            Q<T> un1 = qmul(z0[k], link(Z, g, mu, sp_[k]));
+            Q<T> un0 = qmul_dag(un1, link(Z, g, k, sp_[mu]));
+            Q<T> un2 = qadd(acc, un0);
+            // backward: Z_k(n-k)^+ Z_mu(n-k) Z_k(n-k+mu)
+            un0 = qdag(link(Z, g, k, sm_[k]));
+            un1 = qmul(un0, link(Z, g, mu, sm_[k]));
+            un0 = qmul(un1, link(Z, g, k, shift(g, sm_[k], mu, 1)));
+            acc = qadd(un2, un0);
+        }
+        Q<T> S = qscale(acc, sp);
+        res[mu] = qproject(qadd(z0[mu], S));
+    }
+    // stores last: no store separates the loads the directions share
+#pragma unroll
+    for (int mu = MU0; mu <= MU1; mu++) qstore<T>(O.p[mu][p], i, res[mu]);
 }
 
 // ---------------------------------------------------------------------------

@@ -91,8 +91,7 @@ hipError_t Launch<T>::plaquette(hipStream_t s, const Geo& g, const RawPlanes& P,
 template <typename T>
 hipError_t Launch<T>::ape(hipStream_t s, const Geo& g, const RawPlanes& in, const RawPlanes& out, int mode, double sp)
 {
-    int nmu = (mode == 0) ? 3 : (mode == 1 ? 4 : 1);
-    dim3 grid(grid_x(g), 2 * nmu), block(BLOCK);
+    dim3 grid(grid_x(g), 2), block(BLOCK);
     Planes<T> Z = planes<T>(in), O = planes<T>(out);
     if (mode == 0) hipLaunchKernelGGL((k_ape<T, 0>), grid, block, 0, s, Z, O, g, (T)sp);
     else if (mode == 1) hipLaunchKernelGGL((k_ape<T, 1>), grid, block, 0, s, Z, O, g, (T)sp);

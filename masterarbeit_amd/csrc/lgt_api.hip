@@ -86,6 +86,7 @@ struct lgt_engine {
     void* staging = nullptr;
     size_t staging_bytes = 0;
     int profile = 0;
+    int order = -1;   // tile order of every kernel; -1: per kernel (see kgeo)
     std::vector<ProfEvent> events;
     double prof_ms[LGT_K_COUNT] = {0};
     int64_t prof_n[LGT_K_COUNT] = {0};
@@ -190,6 +191,20 @@ int tile_order_ok(const Geo& g, int order)
 {
     if ((order & 2) && ((g.vh / (uint32_t)g.L[0]) % 256u) != 0) order &= ~2;
     return order;
+}
+
+// Geometry of one launch with its tile order.  Measured on MI355X at 48^4:
+// every plane kernel is fastest with XCD-contiguous tiles (order 1) except the
+// time-direction update sub-sweeps, whose neighbours lie in the adjacent time
+// slices: FP32 time-fastest + XCD-contiguous (3), FP64 plain round-robin (0).
+Geo kgeo(const lgt_engine* e, int cls, int mu)
+{
+    Geo g = e->geo;
+    int order = 1;
+    if (e->order >= 0) order = e->order;
+    else if ((cls == LGT_K_HEATBATH || cls == LGT_K_OVERRELAX) && mu == 0) order = (e->prec == LGT_FP32) ? 3 : 0;
+    g.order = tile_order_ok(g, order);
+    return g;
 }
 
 // reduces `nrows` rows of `ncols` double-doubles to host (hi, lo) pairs
@@ -306,7 +321,11 @@ int lgt_create(const int extents[4], int precision, uint64_t seed, lgt_engine** 
         vol *= extents[d];
     }
     if (extents[3] < 4) return fail(LGT_EINVAL, "lgt_create: extent[3] = %d must be >= 4", extents[3]);
-    if (vol / 2 >= (int64_t(1) << 31)) return fail(LGT_EINVAL, "lgt_create: volume %lld too large", (long long)vol);
+    // kernels address both parity planes of a direction with 32-bit byte offsets
+    const int64_t lb = (precision == LGT_FP64) ? 32 : 16;
+    if (vol * lb >= (int64_t(1) << 32))
+        return fail(LGT_EINVAL, "lgt_create: volume %lld too large for %s links on one device", (long long)vol,
+                    precision == LGT_FP64 ? "FP64" : "FP32");
 
     lgt_engine* e = new lgt_engine();
     hipError_t err = hipGetDevice(&e->dev);
@@ -330,7 +349,7 @@ int lgt_create(const int extents[4], int precision, uint64_t seed, lgt_engine** 
     g.dy = make_fastdiv((uint32_t)extents[2]);
     g.dx = make_fastdiv((uint32_t)extents[1]);
     g.dt = make_fastdiv((uint32_t)extents[0]);
-    g.order = tile_order_ok(g, 3);
+    g.order = 0;
     e->link_bytes = (precision == LGT_FP64) ? 32 : 16;
 
     err = hipStreamCreateWithFlags(&e->stream, hipStreamNonBlocking);
@@ -382,8 +401,8 @@ int lgt_set_option(lgt_engine* e, int option, int value)
     if (!e) return fail(LGT_EINVAL, "lgt_set_option: null engine");
     switch (option) {
     case LGT_OPT_TILE_ORDER:
-        if (value < 0 || value > 3) return fail(LGT_EINVAL, "lgt_set_option: tile order %d", value);
-        e->geo.order = tile_order_ok(e->geo, value);
+        if (value < -1 || value > 3) return fail(LGT_EINVAL, "lgt_set_option: tile order %d", value);
+        e->order = value;
         return LGT_OK;
     case LGT_OPT_PROFILE:
         if (!value) TRY(flush_profile(e));
@@ -448,7 +467,7 @@ int lgt_heatbath_subsweep(lgt_engine* e, int mu, int parity, double beta, double
         ap = e->partial;
     }
     TRY(run(e, LGT_K_HEATBATH, "heatbath", [&] {
-        return DISPATCH(e, heatbath(e->stream, e->geo, e->cur, mu, parity, beta, e->seed, e->sweep, ap));
+        return DISPATCH(e, heatbath(e->stream, kgeo(e, LGT_K_HEATBATH, mu), e->cur, mu, parity, beta, e->seed, e->sweep, ap));
     }));
     if (acc_sum) TRY(reduce_to_host(e, 1, (int)nbx(e), acc_sum));
     return LGT_OK;
@@ -459,7 +478,7 @@ int lgt_overrelax_subsweep(lgt_engine* e, int mu, int parity)
     if (!e || mu < 0 || mu > 3 || parity < 0 || parity > 1) return fail(LGT_EINVAL, "lgt_overrelax_subsweep: mu/parity");
     TRY(set_dev(e));
     return run(e, LGT_K_OVERRELAX, "overrelax",
-               [&] { return DISPATCH(e, overrelax(e->stream, e->geo, e->cur, mu, parity)); });
+               [&] { return DISPATCH(e, overrelax(e->stream, kgeo(e, LGT_K_OVERRELAX, mu), e->cur, mu, parity)); });
 }
 
 int lgt_project(lgt_engine* e)
@@ -494,7 +513,7 @@ int lgt_update(lgt_engine* e, int nsweeps, int n_or, double beta, double* accept
             for (int p = 0; p < 2; p++) {
                 void* ap = acceptance ? (char*)e->partial + sizeof(double) * 2 * (size_t)nb * (mu * 2 + p) : nullptr;
                 TRY(run(e, LGT_K_HEATBATH, "heatbath", [&] {
-                    return DISPATCH(e, heatbath(e->stream, e->geo, e->cur, mu, p, beta, e->seed, e->sweep, ap));
+                    return DISPATCH(e, heatbath(e->stream, kgeo(e, LGT_K_HEATBATH, mu), e->cur, mu, p, beta, e->seed, e->sweep, ap));
                 }));
             }
         if (acceptance) TRY(reduce_to_host(e, 1, (int)(8 * nb), &acc_rows[s]));
@@ -502,7 +521,7 @@ int lgt_update(lgt_engine* e, int nsweeps, int n_or, double beta, double* accept
             for (int mu = 0; mu < 4; mu++)
                 for (int p = 0; p < 2; p++)
                     TRY(run(e, LGT_K_OVERRELAX, "overrelax",
-                            [&] { return DISPATCH(e, overrelax(e->stream, e->geo, e->cur, mu, p)); }));
+                            [&] { return DISPATCH(e, overrelax(e->stream, kgeo(e, LGT_K_OVERRELAX, mu), e->cur, mu, p)); }));
         if (n_or == 0) TRY(run(e, LGT_K_PROJECT, "project", [&] { return DISPATCH(e, project(e->stream, e->geo, e->cur)); }));
         e->sweep++;
     }
@@ -522,7 +541,7 @@ int lgt_plaquette(lgt_engine* e, double* plaq)
     const uint32_t nb = nbx(e);
     TRY(ensure(e, &e->partial, &e->partial_bytes, sizeof(double) * 2 * 2 * (size_t)nb, "partials"));
     TRY(run(e, LGT_K_PLAQUETTE, "plaquette",
-            [&] { return DISPATCH(e, plaquette(e->stream, e->geo, e->cur, e->partial)); }));
+            [&] { return DISPATCH(e, plaquette(e->stream, kgeo(e, LGT_K_PLAQUETTE, 0), e->cur, e->partial)); }));
     double s = 0.0;
     TRY(reduce_to_host(e, 1, (int)(2 * nb), &s));
     *plaq = s / (double)(2 * 6 * e->vol);
@@ -551,7 +570,7 @@ int lgt_ape(lgt_engine* e, int mode, int nsteps, double sp)
     }
     int mu_lo = (mode == LGT_APE_SPATIAL) ? 1 : 0, mu_hi = (mode == LGT_APE_TEMPORAL) ? 0 : 3;
     for (int s = 0; s < nsteps; s++) {
-        TRY(run(e, LGT_K_APE, "ape", [&] { return DISPATCH(e, ape(e->stream, e->geo, e->cur, e->alt, mode, sp)); }));
+        TRY(run(e, LGT_K_APE, "ape", [&] { return DISPATCH(e, ape(e->stream, kgeo(e, LGT_K_APE, 0), e->cur, e->alt, mode, sp)); }));
         for (int mu = mu_lo; mu <= mu_hi; mu++)
             for (int p = 0; p < 2; p++) std::swap(e->cur.p[mu][p], e->alt.p[mu][p]);
     }
@@ -607,7 +626,7 @@ int lgt_wilson_loops(lgt_engine* e, int nloop, const int* R, const int* T, doubl
     HIP_TRY(hipMemcpyAsync(dslots, slots.data(), sizeof(int) * slots.size(), hipMemcpyHostToDevice, e->stream),
             "hipMemcpyAsync");
     TRY(run(e, LGT_K_WLOOP, "wilson loops", [&] {
-        return DISPATCH(e, wloops(e->stream, e->geo, e->cur, dgroups, (int)groups.size(), dslots, e->partial, ncols));
+        return DISPATCH(e, wloops(e->stream, kgeo(e, LGT_K_WLOOP, 0), e->cur, dgroups, (int)groups.size(), dslots, e->partial, ncols));
     }));
     std::vector<double> sums(nuniq);
     TRY(reduce_to_host(e, nuniq, ncols, sums.data()));
@@ -627,7 +646,7 @@ int lgt_qtop(lgt_engine* e, int kind, double* q, double* abs_sum, double* densit
         dq = (double*)e->staging;
     }
     TRY(run(e, LGT_K_QTOP, "qtop",
-            [&] { return DISPATCH(e, qtop(e->stream, e->geo, e->cur, kind, e->partial, dq)); }));
+            [&] { return DISPATCH(e, qtop(e->stream, kgeo(e, LGT_K_QTOP, 0), e->cur, kind, e->partial, dq)); }));
     double s[2];
     TRY(reduce_to_host(e, 2, (int)(2 * nb), s));
     const double PI = 3.14159265358979323846;
@@ -668,7 +687,7 @@ int lgt_ape_vjp_forward(lgt_engine* e, int nsteps, double sp, int segment)
     }
     for (int n = 0; n < nsteps; n++) {
         if (n % seg == 0) TRY(copy_spatial(e, v.ckP[n / seg], e->cur));
-        TRY(run(e, LGT_K_APE, "ape", [&] { return DISPATCH(e, ape(e->stream, e->geo, e->cur, e->alt, 0, sp)); }));
+        TRY(run(e, LGT_K_APE, "ape", [&] { return DISPATCH(e, ape(e->stream, kgeo(e, LGT_K_APE, 0), e->cur, e->alt, 0, sp)); }));
         for (int mu = 1; mu < 4; mu++)
             for (int p = 0; p < 2; p++) std::swap(e->cur.p[mu][p], e->alt.p[mu][p]);
     }
@@ -694,7 +713,7 @@ int lgt_wloop_sum(lgt_engine* e, int R, int T, double sum[2])
     HIP_TRY(hipMemcpyAsync(dg, &gh, sizeof(gh), hipMemcpyHostToDevice, e->stream), "hipMemcpyAsync");
     HIP_TRY(hipMemcpyAsync(ds, slots.data(), sizeof(int) * T, hipMemcpyHostToDevice, e->stream), "hipMemcpyAsync");
     TRY(run(e, LGT_K_WLOOP, "wilson loops",
-            [&] { return DISPATCH(e, wloops(e->stream, e->geo, e->cur, dg, 1, ds, e->partial, ncols)); }));
+            [&] { return DISPATCH(e, wloops(e->stream, kgeo(e, LGT_K_WLOOP, 0), e->cur, dg, 1, ds, e->partial, ncols)); }));
     return reduce_to_host_dd(e, 1, ncols, sum);
 }
 
@@ -716,7 +735,7 @@ int lgt_wloop_grad(lgt_engine* e, int R, int T, double* grad)
     VjpState& v = e->vjp;
     if (!v.gobj) TRY(alloc_field(e, &v.gobj, &v.gobjP));
     TRY(run(e, LGT_K_WLOOP_GRAD, "wilson-loop gradient",
-            [&] { return DISPATCH(e, wloop_grad(e->stream, e->geo, e->cur, R, T, v.gobjP)); }));
+            [&] { return DISPATCH(e, wloop_grad(e->stream, kgeo(e, LGT_K_WLOOP_GRAD, 0), e->cur, R, T, v.gobjP)); }));
     v.cot = true;
     if (grad) return field_to_host(e, v.gobjP, grad);
     HIP_TRY(hipStreamSynchronize(e->stream), "hipStreamSynchronize");
@@ -758,7 +777,7 @@ int lgt_ape_vjp_backward(lgt_engine* e, const double* cotangent, double* grad)
         for (int n = n0; n < n1; n++) {
             const RawPlanes& in = (n == n0) ? v.ckP[s] : v.segUP[n - n0 - 1];
             TRY(run(e, LGT_K_APE_SAVE, "ape recompute", [&] {
-                return DISPATCH(e, ape_save(e->stream, e->geo, in, v.segUP[n - n0], v.segMN[n - n0], sp));
+                return DISPATCH(e, ape_save(e->stream, kgeo(e, LGT_K_APE_SAVE, 0), in, v.segUP[n - n0], v.segMN[n - n0], sp));
             }));
         }
         const RawPlanes& Utop = (s == v.nck - 1) ? e->cur : v.ckP[s + 1];
@@ -772,7 +791,7 @@ int lgt_ape_vjp_backward(lgt_engine* e, const double* cotangent, double* grad)
             const RawPlanes& Z = (n == n0) ? v.ckP[s] : v.segUP[n - n0 - 1];
             const RawNorms* mprev = (n > n0) ? &v.segMN[n - 1 - n0] : nullptr;
             TRY(run(e, LGT_K_VJP, "adjoint step", [&] {
-                return DISPATCH(e, vjp_step(e->stream, e->geo, Z, v.gP[h], mprev, v.gP[1 - h], sp));
+                return DISPATCH(e, vjp_step(e->stream, kgeo(e, LGT_K_VJP, 0), Z, v.gP[h], mprev, v.gP[1 - h], sp));
             }));
             h = 1 - h;
         }

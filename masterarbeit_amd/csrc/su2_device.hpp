@@ -191,7 +191,10 @@ __device__ __forceinline__ Site shift(const Geo& g, Site s, int mu, int step)
     return s;
 }
 
-// link planes of one field: p[mu][parity] -> VOL/2 links
+// link planes of one field: p[mu][parity] -> VOL/2 links.  The two parity
+// planes of a direction are adjacent, p[mu][1] == p[mu][0] + VOL/2, so a link
+// of any parity is one 32-bit byte offset from p[mu][0] (the host keeps
+// 2 * VOL/2 * sizeof(link) < 2^32).
 template <typename T> struct Planes {
     typename Store<T>::V* p[4][2];
 };
@@ -199,7 +202,10 @@ template <typename T> struct Planes {
 template <typename T>
 __device__ __forceinline__ Q<T> link(const Planes<T>& P, const Geo& g, int mu, const Site& s)
 {
-    return qload<T>(P.p[mu][parity(s)], cb_index(g, s));
+    const uint32_t off = ((uint32_t)parity(s) * g.vh + cb_index(g, s)) * (uint32_t)sizeof(typename Store<T>::V);
+    const typename Store<T>::V* q = (const typename Store<T>::V*)((const char*)P.p[mu][0] + off);
+    typename Store<T>::V v = *q;
+    return Q<T>{v.x, v.y, v.z, v.w};
 }
 
 // ---------------------------------------------------------------------------

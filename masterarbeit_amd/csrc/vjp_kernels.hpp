@@ -25,11 +25,6 @@ namespace lgt {
 // scalar planes of |M| (one value per spatial link; direction 0 unused)
 template <typename T> struct NormPlanes { T* p[4][2]; };
 
-template <typename T>
-__device__ __forceinline__ T nload(const NormPlanes<T>& N, const Geo& g, int mu, const Site& s)
-{
-    return N.p[mu][parity(s)][cb_index(g, s)];
-}
 
 // forward spatial APE step that also records |M| (values equal k_ape's)
 template <typename T>
