@@ -106,6 +106,33 @@ def time_stream(e, prec, L, reps=10):
     return {"bytes_per_launch": nbytes, "kernel_ms_avg": ms / n, "achieved_GBs": nbytes / (ms / n * 1e-3) / 1e9}
 
 
+def time_vjp(lgt, L, prec, seed, beta, nsteps=400, sp=0.25, R=5, T=5, therm=50):
+    """Adjoint of N spatial APE steps (alpha = 0.5, sp = 0.25) with the loop
+    objective L = sum_x sum_d 2 Re W_{RxT}: forward with sqrt(N) checkpoints,
+    then the backward pass (recompute + adjoint steps) for the objective's
+    gradient.  The ratio is backward / forward wall time."""
+    e = lgt.Engine(L, prec, seed)
+    e.init_field(1)
+    e.update(therm, 1, beta)
+    e.sync()
+    t0 = time.perf_counter()
+    e.ape_vjp_forward(nsteps, sp)
+    e.sync()
+    t_fwd = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    e.wloop_grad(R, T, host=False)
+    e.sync()
+    t_obj = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    e.ape_vjp_backward(grad=False)
+    e.sync()
+    t_bwd = time.perf_counter() - t0
+    out = {"nsteps": nsteps, "segment": int(np.ceil(np.sqrt(nsteps))), "forward_s": t_fwd, "backward_s": t_bwd,
+           "objective_grad_s": t_obj, "vjp_over_forward": t_bwd / t_fwd, "device_bytes": e.device_bytes()}
+    e.close()
+    return out
+
+
 def time_observables(e):
     out = {}
     t0 = time.perf_counter()
@@ -166,6 +193,7 @@ def main():
     ap.add_argument("--seed", type=int, default=7)
     ap.add_argument("--nplaq", type=int, default=50)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--vjp-steps", type=int, default=400)
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on exactly one GPU; --gpus {args.gpus} is not supported (the engine has no "
@@ -192,6 +220,10 @@ def main():
     log(f"[bench] fp64 sweep: {r64['ms_per_step']:.3f} ms/step, {r64['link_updates_per_s']:.4g} link updates/s")
     ape64 = time_ape(e64, 0, L)
     obs64 = time_observables(e64)
+    log("[bench] APE-400 adjoint (fp64) ...")
+    vjp64 = time_vjp(lgt, L, lgt.FP64, args.seed, args.beta, nsteps=args.vjp_steps)
+    log(f"[bench] fp64 VJP/forward = {vjp64['vjp_over_forward']:.2f} (forward {vjp64['forward_s']:.3f} s)")
+    vjp32 = time_vjp(lgt, L, lgt.FP32, args.seed, args.beta, nsteps=args.vjp_steps)
 
     # dominant kernel of the headline chain
     k = r32["kernels"]
@@ -259,6 +291,7 @@ def main():
             "n": len(p32),
         },
         "ape_spatial": {"fp32": ape32, "fp64": ape64},
+        "ape_vjp": {"fp64": vjp64, "fp32": vjp32},
         "stream_pass": stream32,
         "observables_s": {"fp32": obs32, "fp64": obs64},
         "device_bytes": {"fp32": r32["device_bytes"], "fp64": r64["device_bytes"]},

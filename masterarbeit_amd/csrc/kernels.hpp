@@ -492,9 +492,12 @@ __device__ __forceinline__ Q<T> ape_staple(const Planes<T>& Z, const Geo& g, con
 // Z_a(n-a+b)) are loaded once: 24 loads for the 3 spatial links instead of
 // 36, 44 instead of 72 for all four.  Each staple is still summed in the
 // reference's order (k ascending, forward then backward term).
-// grid: x over cb sites, y = parity
+// grid: x over cb sites, y = parity.  With Mn.p[1][0] set (spatial mode),
+// |M| = |Z + sp S| of every smeared link is stored too (for the adjoint).
+template <typename T> struct NormPlanes { T* p[4][2]; };
+
 template <typename T, int MODE>
-__global__ void __launch_bounds__(BLOCK) k_ape(Planes<T> Z, Planes<T> O, Geo g, T sp)
+__global__ void __launch_bounds__(BLOCK) k_ape(Planes<T> Z, Planes<T> O, Geo g, T sp, NormPlanes<T> Mn)
 {
     constexpr int MU0 = (MODE == 0) ? 1 : 0, MU1 = (MODE == 2) ? 0 : 3;
     uint32_t i = tile_base(g) + threadIdx.x;
@@ -504,6 +507,7 @@ __global__ void __launch_bounds__(BLOCK) k_ape(Planes<T> Z, Planes<T> O, Geo g, 
     // directions that enter the staples
     constexpr int K0 = (MODE == 1) ? 0 : 1;
     Q<T> z0[4], res[4];
+    T nrm[4];
     Site sp_[4], sm_[4];
 #pragma unroll
     for (int b = 0; b < 4; b++) {
@@ -533,11 +537,16 @@ __global__ void __launch_bounds__(BLOCK) k_ape(Planes<T> Z, Planes<T> O, Geo g, 
             acc = qadd(un2, un0);
         }
         Q<T> S = qscale(acc, sp);
-        res[mu] = qproject(qadd(z0[mu], S));
+        Q<T> M = qadd(z0[mu], S);
+        nrm[mu] = qdet_sqrt(M);
+        res[mu] = qproject(M);
     }
     // stores last: no store separates the loads the directions share
 #pragma unroll
-    for (int mu = MU0; mu <= MU1; mu++) qstore<T>(O.p[mu][p], i, res[mu]);
+    for (int mu = MU0; mu <= MU1; mu++) {
+        if (MODE == 0 && Mn.p[1][0]) Mn.p[mu][p][i] = nrm[mu];
+        qstore<T>(O.p[mu][p], i, res[mu]);
+    }
 }
 
 // ---------------------------------------------------------------------------

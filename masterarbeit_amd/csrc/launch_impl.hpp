@@ -93,9 +93,10 @@ hipError_t Launch<T>::ape(hipStream_t s, const Geo& g, const RawPlanes& in, cons
 {
     dim3 grid(grid_x(g), 2), block(BLOCK);
     Planes<T> Z = planes<T>(in), O = planes<T>(out);
-    if (mode == 0) hipLaunchKernelGGL((k_ape<T, 0>), grid, block, 0, s, Z, O, g, (T)sp);
-    else if (mode == 1) hipLaunchKernelGGL((k_ape<T, 1>), grid, block, 0, s, Z, O, g, (T)sp);
-    else hipLaunchKernelGGL((k_ape<T, 2>), grid, block, 0, s, Z, O, g, (T)sp);
+    NormPlanes<T> none{};
+    if (mode == 0) hipLaunchKernelGGL((k_ape<T, 0>), grid, block, 0, s, Z, O, g, (T)sp, none);
+    else if (mode == 1) hipLaunchKernelGGL((k_ape<T, 1>), grid, block, 0, s, Z, O, g, (T)sp, none);
+    else hipLaunchKernelGGL((k_ape<T, 2>), grid, block, 0, s, Z, O, g, (T)sp, none);
     return hipGetLastError();
 }
 
@@ -132,8 +133,8 @@ template <typename T>
 hipError_t Launch<T>::ape_save(hipStream_t s, const Geo& g, const RawPlanes& in, const RawPlanes& out,
                                const RawNorms& mn, double sp)
 {
-    hipLaunchKernelGGL(k_ape_save<T>, dim3(grid_x(g), 6), dim3(BLOCK), 0, s, planes<T>(in), planes<T>(out), norms<T>(mn),
-                       g, (T)sp);
+    hipLaunchKernelGGL((k_ape<T, 0>), dim3(grid_x(g), 2), dim3(BLOCK), 0, s, planes<T>(in), planes<T>(out), g, (T)sp,
+                       norms<T>(mn));
     return hipGetLastError();
 }
 
@@ -151,7 +152,7 @@ hipError_t Launch<T>::vjp_step(hipStream_t s, const Geo& g, const RawPlanes& Z, 
                                const RawNorms* mprev, const RawPlanes& out, double sp)
 {
     RawNorms none{};
-    hipLaunchKernelGGL(k_vjp_step<T>, dim3(grid_x(g), 6), dim3(BLOCK), 0, s, planes<T>(Z), planes<T>(Hin),
+    hipLaunchKernelGGL(k_vjp_step<T>, dim3(grid_x(g), 2), dim3(BLOCK), 0, s, planes<T>(Z), planes<T>(Hin),
                        norms<T>(mprev ? *mprev : none), mprev ? 1 : 0, planes<T>(out), g, (T)sp);
     return hipGetLastError();
 }

@@ -22,25 +22,6 @@
 
 namespace lgt {
 
-// scalar planes of |M| (one value per spatial link; direction 0 unused)
-template <typename T> struct NormPlanes { T* p[4][2]; };
-
-
-// forward spatial APE step that also records |M| (values equal k_ape's)
-template <typename T>
-__global__ void __launch_bounds__(BLOCK) k_ape_save(Planes<T> Z, Planes<T> O, NormPlanes<T> Mn, Geo g, T sp)
-{
-    uint32_t i = tile_base(g) + threadIdx.x;
-    if (i >= g.vh) return;
-    int p = blockIdx.y & 1;
-    int mu = 1 + (blockIdx.y >> 1);
-    Site s = cb_site(g, i, p);
-    Q<T> S = qscale(ape_staple(Z, g, s, mu, 0), sp);
-    Q<T> M = qadd(qload<T>(Z.p[mu][p], i), S);
-    T m = qdet_sqrt(M);
-    Mn.p[mu][p][i] = m;
-    qstore<T>(O.p[mu][p], i, qproject(M));
-}
 
 // H = (G - <G,U> U) / |M| on the spatial links
 template <typename T>
@@ -67,6 +48,9 @@ __device__ __forceinline__ Q<T> mul3(const Q<T>& a, const Q<T>& b, const Q<T>& c
 // its output (H above).  Writes G = dL/dZ for the spatial links, or, when
 // Mprev is given (|M| of the step before), directly the tangent cotangent of
 // that earlier step: (G - <G,Z> Z)/Mprev  (Z is that step's output).
+// One thread handles the three spatial links of its site, so the gathers the
+// directions share are loaded once (51 instead of 81).
+// grid: x over cb sites, y = parity
 template <typename T>
 __global__ void __launch_bounds__(BLOCK) k_vjp_step(Planes<T> Z, Planes<T> Hin, NormPlanes<T> Mprev, int has_prev,
                                                     Planes<T> Out, Geo g, T sp)
@@ -74,34 +58,43 @@ __global__ void __launch_bounds__(BLOCK) k_vjp_step(Planes<T> Z, Planes<T> Hin, 
 #pragma clang fp contract(fast)
     uint32_t i = tile_base(g) + threadIdx.x;
     if (i >= g.vh) return;
-    int p = blockIdx.y & 1;
-    int a = 1 + (blockIdx.y >> 1);
+    const int p = blockIdx.y;
     Site y = cb_site(g, i, p);
-    Site ya = shift(g, y, a, 1);
-    Q<T> acc = qzero<T>();
+    Site yp[4], ym[4];
 #pragma unroll
     for (int b = 1; b < 4; b++) {
-        if (b == a) continue;
-        Site yb = shift(g, y, b, 1), ymb = shift(g, y, b, -1);
-        Site ymba = shift(g, ymb, a, 1);
-        Q<T> Za_yb = link(Z, g, a, yb), Zb_ya = link(Z, g, b, ya), Zb_y = link(Z, g, b, y);
-        Q<T> Zb_ymb = link(Z, g, b, ymb), Za_ymb = link(Z, g, a, ymb), Zb_ymba = link(Z, g, b, ymba);
-        Q<T> Hb_y = link(Hin, g, b, y), Ha_ymb = link(Hin, g, a, ymb), Hb_ymb = link(Hin, g, b, ymb);
-        Q<T> Hb_ya = link(Hin, g, b, ya), Ha_yb = link(Hin, g, a, yb), Hb_ymba = link(Hin, g, b, ymba);
-        acc = qadd(acc, mul3(Hb_y, Za_yb, qdag(Zb_ya)));            // t1
-        acc = qadd(acc, mul3(qdag(Zb_ymb), Ha_ymb, Zb_ymba));       // t2
-        acc = qadd(acc, mul3(qdag(Hb_ymb), Za_ymb, Zb_ymba));       // t3
-        acc = qadd(acc, mul3(Zb_y, Za_yb, qdag(Hb_ya)));            // t4
-        acc = qadd(acc, mul3(Zb_y, Ha_yb, qdag(Zb_ya)));            // t5
-        acc = qadd(acc, mul3(qdag(Zb_ymb), Za_ymb, Hb_ymba));       // t6
+        yp[b] = shift(g, y, b, 1);
+        ym[b] = shift(g, y, b, -1);
     }
-    Q<T> G = qadd(qload<T>(Hin.p[a][p], i), qscale(acc, sp));
-    if (has_prev) {
-        Q<T> z = qload<T>(Z.p[a][p], i);
-        T c = qdot(G, z), m = Mprev.p[a][p][i];
-        G = Q<T>{(G.c0 - c * z.c0) / m, (G.c1 - c * z.c1) / m, (G.c2 - c * z.c2) / m, (G.c3 - c * z.c3) / m};
+    Q<T> res[4];
+#pragma unroll
+    for (int a = 1; a < 4; a++) {
+        Q<T> acc = qzero<T>();
+#pragma unroll
+        for (int b = 1; b < 4; b++) {
+            if (b == a) continue;
+            Site ymba = shift(g, ym[b], a, 1);
+            Q<T> Za_yb = link(Z, g, a, yp[b]), Zb_ya = link(Z, g, b, yp[a]), Zb_y = link(Z, g, b, y);
+            Q<T> Zb_ymb = link(Z, g, b, ym[b]), Za_ymb = link(Z, g, a, ym[b]), Zb_ymba = link(Z, g, b, ymba);
+            Q<T> Hb_y = link(Hin, g, b, y), Ha_ymb = link(Hin, g, a, ym[b]), Hb_ymb = link(Hin, g, b, ym[b]);
+            Q<T> Hb_ya = link(Hin, g, b, yp[a]), Ha_yb = link(Hin, g, a, yp[b]), Hb_ymba = link(Hin, g, b, ymba);
+            acc = qadd(acc, mul3(Hb_y, Za_yb, qdag(Zb_ya)));            // t1
+            acc = qadd(acc, mul3(qdag(Zb_ymb), Ha_ymb, Zb_ymba));       // t2
+            acc = qadd(acc, mul3(qdag(Hb_ymb), Za_ymb, Zb_ymba));       // t3
+            acc = qadd(acc, mul3(Zb_y, Za_yb, qdag(Hb_ya)));            // t4
+            acc = qadd(acc, mul3(Zb_y, Ha_yb, qdag(Zb_ya)));            // t5
+            acc = qadd(acc, mul3(qdag(Zb_ymb), Za_ymb, Hb_ymba));       // t6
+        }
+        Q<T> G = qadd(link(Hin, g, a, y), qscale(acc, sp));
+        if (has_prev) {
+            Q<T> z = link(Z, g, a, y);
+            T c = qdot(G, z), m = Mprev.p[a][p][i];
+            G = Q<T>{(G.c0 - c * z.c0) / m, (G.c1 - c * z.c1) / m, (G.c2 - c * z.c2) / m, (G.c3 - c * z.c3) / m};
+        }
+        res[a] = G;
     }
-    qstore<T>(Out.p[a][p], i, G);
+#pragma unroll
+    for (int a = 1; a < 4; a++) qstore<T>(Out.p[a][p], i, res[a]);
 }
 
 // ---------------------------------------------------------------------------

@@ -116,3 +116,38 @@ def test_vjp_matches_gpu_finite_differences(lgt, prec, rtol, eps):
             fd = ((vals[0][0] - vals[1][0]) + (vals[0][1] - vals[1][1])) / (2 * eps)
             an = float(np.dot(G[n, mu], F.qmul(X, U[n, mu])))
             assert an == pytest.approx(fd, rel=rtol, abs=1e-7 if prec == 0 else 1e-3), (n, mu, an, fd)
+
+
+def test_vjp_400_steps_48_finite_differences(lgt):
+    """The 48^4 case: FP64, thermalised from seed 11, 400 spatial APE steps
+    (alpha = 0.5), L = sum of 5x5 loops, sqrt(N) = 20 checkpoints; central
+    differences (eps = 1e-4) along 16 random su(2) directions on random links,
+    with the loop sums kept in double-double."""
+    L = (48, 48, 48, 48)
+    R = T = 5
+    with lgt.Engine(L, 0, 11) as e:
+        e.init_field(1)
+        e.update(100, 1, 3.09)
+        U = e.download()
+        e.ape_vjp_forward(400, 0.25)
+        e.wloop_grad(R, T, host=False)
+        G = e.ape_vjp_backward()
+        rng = np.random.default_rng(11)
+        worst = 0.0
+        for _ in range(16):
+            n, mu = int(rng.integers(U.shape[0])), int(rng.integers(4))
+            X = np.concatenate([[0.0], rng.normal(size=3)])
+            vals = []
+            for sgn in (1, -1):
+                old = U[n, mu].copy()
+                U[n, mu] = F.qmul(su2_exp(X, sgn * 1e-4), old)
+                e.upload(U)
+                U[n, mu] = old
+                e.ape(0, 400, 0.25)
+                vals.append(e.wloop_sum(R, T))
+            fd = ((vals[0][0] - vals[1][0]) + (vals[0][1] - vals[1][1])) / 2e-4
+            an = float(np.dot(G[n, mu], F.qmul(X, U[n, mu])))
+            rel = abs(an - fd) / max(abs(fd), 1e-30)
+            worst = max(worst, rel)
+            assert rel < 1e-6 or abs(an - fd) < 1e-9, (n, mu, an, fd)
+        print(f"48^4 APE-400 adjoint vs finite differences: worst relative deviation {worst:.2e}")
