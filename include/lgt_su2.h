@@ -50,10 +50,11 @@ typedef struct lgt_engine lgt_engine;
 #define LGT_Q_CLOVER 1     /* meas_topologicalcharge(): 1x1 clover, :391-409 */
 
 /* engine options (lgt_set_option) */
-#define LGT_OPT_TILE_ORDER 0 /* order of the 256-site tiles: -1 (default) = tuned per kernel;
+#define LGT_OPT_TILE_ORDER 0 /* order of the 256-site tiles: -1 (default) = tuned;
                                 else flags 1 = each XCD takes a contiguous share of the
-                                tiles, 2 = time coordinate fastest (ignored unless a time
-                                slice holds whole tiles).  Results do not depend on it. */
+                                tiles, 2 = time coordinate fastest, 4 = time fastest within
+                                groups of spatial tiles (2 and 4 apply when a time slice
+                                holds whole tiles).  Results do not depend on it. */
 #define LGT_OPT_PROFILE 1    /* 1: time every launch with HIP events */
 
 /* kernel classes of the launch profile (lgt_profile_read) */

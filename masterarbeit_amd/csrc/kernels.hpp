@@ -25,11 +25,22 @@ constexpr double PI_D = 3.14159265358979323846;   // = 2 asin(1), LGT_Code/modul
 //         consecutive t, so the +-t neighbours (a whole time slice away in
 //         memory) are read while still in L2.  Needs the cb sites of one time
 //         slice to be a multiple of 256; the host clears the bit otherwise.
+//  bit 2: grouped: time fastest within groups of g.gs consecutive spatial
+//         tiles, group after group, so that the +-x neighbours (a few tiles
+//         away in the same time slice) and the +-t neighbours (g.gs tiles
+//         away) both stay close in the sequence.
 // Returns the first cb index of the block's tile.
 __device__ __forceinline__ uint32_t tile_base(const Geo& g)
 {
     uint32_t b = blockIdx.x, nb = gridDim.x;
     if ((g.order & 1) && (nb & 7u) == 0) b = (b & 7u) * (nb >> 3) + (b >> 3);
+    if (g.order & 4) {
+        uint32_t grp = fdiv(g.dgt, b);
+        uint32_t r = b - grp * ((uint32_t)g.gs * (uint32_t)g.L[0]);
+        uint32_t t = fdiv(g.dgs, r);
+        uint32_t s = grp * (uint32_t)g.gs + (r - t * (uint32_t)g.gs);
+        return t * (g.vh / (uint32_t)g.L[0]) + s * 256u;
+    }
     if (g.order & 2) {
         uint32_t s = fdiv(g.dt, b);
         uint32_t t = b - s * (uint32_t)g.L[0];

@@ -186,23 +186,24 @@ int flush_profile(lgt_engine* e)
 
 uint32_t nbx(const lgt_engine* e) { return grid_x(e->geo); }
 
-// the time-fastest order needs whole 256-site tiles per time slice
+// the time-fastest and grouped orders need whole 256-site tiles per time slice
 int tile_order_ok(const Geo& g, int order)
 {
+    if ((order & 4) && g.gs == 0) order &= ~4;
     if ((order & 2) && ((g.vh / (uint32_t)g.L[0]) % 256u) != 0) order &= ~2;
     return order;
 }
 
 // Geometry of one launch with its tile order.  Measured on MI355X at 48^4:
-// every plane kernel is fastest with XCD-contiguous tiles (order 1) except the
-// time-direction update sub-sweeps, whose neighbours lie in the adjacent time
-// slices: FP32 time-fastest + XCD-contiguous (3), FP64 plain round-robin (0).
+// the grouped, XCD-contiguous order (5) serves the staple gathers of every
+// direction best; without whole tiles per time slice the XCD-contiguous
+// linear order (1) is used.
 Geo kgeo(const lgt_engine* e, int cls, int mu)
 {
+    (void)cls;
+    (void)mu;
     Geo g = e->geo;
-    int order = 1;
-    if (e->order >= 0) order = e->order;
-    else if ((cls == LGT_K_HEATBATH || cls == LGT_K_OVERRELAX) && mu == 0) order = (e->prec == LGT_FP32) ? 3 : 0;
+    int order = (e->order >= 0) ? e->order : (g.gs ? 5 : 1);
     g.order = tile_order_ok(g, order);
     return g;
 }
@@ -350,6 +351,18 @@ int lgt_create(const int extents[4], int precision, uint64_t seed, lgt_engine** 
     g.dx = make_fastdiv((uint32_t)extents[1]);
     g.dt = make_fastdiv((uint32_t)extents[0]);
     g.order = 0;
+    // grouped order: the largest divisor <= 27 of the tiles in one time slice
+    g.gs = 0;
+    if ((g.vh / (uint32_t)extents[0]) % 256u == 0) {
+        uint32_t tiles = g.vh / (uint32_t)extents[0] / 256u;
+        for (uint32_t d = 27; d >= 1; d--)
+            if (tiles % d == 0) {
+                g.gs = (int)d;
+                break;
+            }
+        g.dgs = make_fastdiv((uint32_t)g.gs);
+        g.dgt = make_fastdiv((uint32_t)g.gs * (uint32_t)extents[0]);
+    }
     e->link_bytes = (precision == LGT_FP64) ? 32 : 16;
 
     err = hipStreamCreateWithFlags(&e->stream, hipStreamNonBlocking);
@@ -401,7 +414,7 @@ int lgt_set_option(lgt_engine* e, int option, int value)
     if (!e) return fail(LGT_EINVAL, "lgt_set_option: null engine");
     switch (option) {
     case LGT_OPT_TILE_ORDER:
-        if (value < -1 || value > 3) return fail(LGT_EINVAL, "lgt_set_option: tile order %d", value);
+        if (value < -1 || value > 7) return fail(LGT_EINVAL, "lgt_set_option: tile order %d", value);
         e->order = value;
         return LGT_OK;
     case LGT_OPT_PROFILE:

@@ -155,7 +155,9 @@ struct Geo {
     uint32_t stride[4];// cb index strides of t, x, y (z handled apart)
     FastDiv dz, dy, dx;// divisions by hz, L2, L1
     FastDiv dt;        // division by L0 (tile order)
-    int order;         // tile order flags: 1 = XCD-contiguous, 2 = time fastest
+    int order;         // tile order flags: 1 = XCD-contiguous, 2 = time fastest, 4 = grouped
+    int gs;            // spatial tiles per group of the grouped order (0: not available)
+    FastDiv dgs, dgt;  // divisions by gs and gs * L0
 };
 
 struct Site { int c[4]; };

@@ -293,9 +293,9 @@ def test_update_matches_oracle_chain(lgt, L, n_or):
 def test_results_independent_of_tile_order(lgt, prec):
     """Every tile order (XCD-contiguous, time-fastest, both, neither) gives
     bit-identical sweeps, smearing and observables; repeated runs too."""
-    L = (8, 8, 8, 8)       # one time slice = 256 cb sites: the time-fastest order applies
+    L = (8, 16, 16, 16)    # a time slice holds 8 whole tiles: every order applies (groups of 8)
     res = []
-    for order in (-1, 0, 1, 2, 3, -1):
+    for order in (-1, 0, 1, 2, 3, 4, 5, 7, -1):
         with engine(lgt, L, prec, seed=7) as e:
             e.set_option(0, order)
             e.init_field(1)
