@@ -250,46 +250,51 @@ __device__ __forceinline__ Q<double> random_su2_64(Src& src)
     return u;
 }
 
-// localHeatbathUpdate for a given staple (heatbath.c:124-149); returns 1/count.
-// Unold is kept when the bounded trial loop ends without acceptance.
+// localHeatbathUpdate for a given staple (heatbath.c:124-149); returns the
+// trial count (0 on the degenerate branch).  When the bounded trial loop ends
+// without acceptance (alpha -> 0, where the heatbath distribution becomes
+// uniform on SU(2)) the link is drawn uniformly like the degenerate branch.
 template <class Src>
-__device__ __forceinline__ double heatbath_link64(Q<double> A, double beta, Src& src, const Q<double>& Unold,
-                                                  Q<double>& Unew)
+__device__ __forceinline__ int heatbath_link64(Q<double> A, double beta, Src& src, Q<double>& Unew)
 {
     double sqrt_det = qdet_sqrt(A);
     if (sqrt_det <= __DBL_EPSILON__) {
         Unew = random_su2_64(src);
-        return 0.0;
+        return 0;
     }
     A = qdiv(A, sqrt_det);
     double alpha = sqrt_det * beta;
     Q<double> X;
     bool ok;
     int count = kp_draw64(alpha, src, X, ok);
-    Unew = ok ? qmul_dag(X, A) : Unold;
-    return 1.0 / count;
+    Unew = ok ? qmul_dag(X, A) : random_su2_64(src);
+    return count;
 }
 
 // FP32: one Philox call per trial (four 24-bit uniforms), hardware
 // transcendentals (v_log_f32 = log2, v_cos_f32/v_sin_f32 take revolutions),
 // 1 - x0^2 as delta(2 - delta) and sin(acos(c)) as 2 sqrt(a(1-a)).
-__device__ __forceinline__ double heatbath_link32(Q<float> A, float beta, LinkRng& rng, const Q<float>& Unold,
-                                                  Q<float>& Unew)
+__device__ __forceinline__ Q<float> random_su2_32(LinkRng& rng)
+{
+    U32x4 w = rng.next();
+    float r0 = u24(w.x), r1 = u24(w.y), r2 = u24(w.z);
+    Q<float> u;
+    u.c0 = 1.0f - 2.0f * r0;
+    u.c1 = 1.0f - 2.0f * r1;
+    float zw = sqrtf(fmaxf(0.0f, 1.0f - u.c1 * u.c1));
+    u.c2 = zw * __builtin_amdgcn_cosf(r2);
+    u.c3 = zw * __builtin_amdgcn_sinf(r2);
+    zw = sqrtf(fmaxf(0.0f, 1.0f - u.c0 * u.c0));
+    u.c1 *= zw; u.c2 *= zw; u.c3 *= zw;
+    return u;
+}
+
+__device__ __forceinline__ int heatbath_link32(Q<float> A, float beta, LinkRng& rng, Q<float>& Unew)
 {
     float n2 = A.c0 * A.c0 + A.c1 * A.c1 + A.c2 * A.c2 + A.c3 * A.c3;
     if (n2 <= 1e-30f) {
-        U32x4 w = rng.next();
-        float r0 = u24(w.x), r1 = u24(w.y), r2 = u24(w.z);
-        Q<float> u;
-        u.c0 = 1.0f - 2.0f * r0;
-        u.c1 = 1.0f - 2.0f * r1;
-        float zw = sqrtf(fmaxf(0.0f, 1.0f - u.c1 * u.c1));
-        u.c2 = zw * __builtin_amdgcn_cosf(r2);
-        u.c3 = zw * __builtin_amdgcn_sinf(r2);
-        zw = sqrtf(fmaxf(0.0f, 1.0f - u.c0 * u.c0));
-        u.c1 *= zw; u.c2 *= zw; u.c3 *= zw;
-        Unew = u;
-        return 0.0;
+        Unew = random_su2_32(rng);
+        return 0;
     }
     float rn = rsqrtf(n2);
     float a = n2 * rn;                      // |A|
@@ -310,8 +315,8 @@ __device__ __forceinline__ double heatbath_link32(Q<float> A, float beta, LinkRn
         accepted = !(u3 * u3 > 1.0f - 0.5f * delta);
     } while (!accepted && count < KP_MAX_TRIALS);
     if (!accepted) {
-        Unew = Unold;
-        return 1.0 / count;
+        Unew = random_su2_32(rng);
+        return count;
     }
     U32x4 w = rng.next();
     float a0 = u24(w.x), a1 = u24(w.y);
@@ -324,25 +329,24 @@ __device__ __forceinline__ double heatbath_link32(Q<float> A, float beta, LinkRn
     X.c2 = len * st * __builtin_amdgcn_sinf(a0);
     X.c3 = len * ct;
     Unew = qmul_dag(X, V);
-    return 1.0 / count;
+    return count;
 }
 
 template <typename T> struct HB;
 template <> struct HB<double> {
-    __device__ static __forceinline__ double update(const Q<double>& A, double beta, uint64_t seed, uint32_t i,
-                                                    uint32_t tag, uint32_t sweep, const Q<double>& Uo,
-                                                    Q<double>& Un)
+    __device__ static __forceinline__ int update(const Q<double>& A, double beta, uint64_t seed, uint32_t i,
+                                                 uint32_t tag, uint32_t sweep, Q<double>& Un)
     {
         RngSrc64 src(seed, i, tag, sweep);
-        return heatbath_link64(A, beta, src, Uo, Un);
+        return heatbath_link64(A, beta, src, Un);
     }
 };
 template <> struct HB<float> {
-    __device__ static __forceinline__ double update(const Q<float>& A, float beta, uint64_t seed, uint32_t i,
-                                                    uint32_t tag, uint32_t sweep, const Q<float>& Uo, Q<float>& Un)
+    __device__ static __forceinline__ int update(const Q<float>& A, float beta, uint64_t seed, uint32_t i,
+                                                 uint32_t tag, uint32_t sweep, Q<float>& Un)
     {
         LinkRng rng(seed, i, tag, sweep);
-        return heatbath_link32(A, beta, rng, Uo, Un);
+        return heatbath_link32(A, beta, rng, Un);
     }
 };
 
@@ -355,16 +359,16 @@ __global__ void __launch_bounds__(BLOCK) k_heatbath(Planes<T> P, Geo g, int p, T
     uint32_t base = tile_base(g);
     uint32_t tile = base / BLOCK;
     uint32_t i = base + threadIdx.x;
-    double acc = 0.0;
+    int count = 0;
     if (i < g.vh) {
         Site s = cb_site(g, i, p);
         Q<T> A = update_staple<T, MU>(P, g, s, p);
-        Q<T> Uo = qload<T>(P.p[MU][p], i), Un;
-        acc = HB<T>::update(A, beta, seed, i, (uint32_t)(MU | (p << 2) | (KIND_HB << 3)), sweep, Uo, Un);
+        Q<T> Un;
+        count = HB<T>::update(A, beta, seed, i, (uint32_t)(MU | (p << 2) | (KIND_HB << 3)), sweep, Un);
         qstore<T>(P.p[MU][p], i, Un);
     }
     if (acc_partial) {
-        DD s = block_sum_dd(DD{acc, 0.0});
+        DD s = block_sum_dd(DD{count > 0 ? 1.0 / count : 0.0, 0.0});
         if (threadIdx.x == 0) acc_partial[tile] = s;
     }
 }
@@ -416,11 +420,10 @@ static __global__ void __launch_bounds__(BLOCK) k_hb_harness(int n, const double
     if (i >= n) return;
     BufSrc src{uniforms + offsets[i], lens[i], 0};
     Q<double> A{staples[4 * i], staples[4 * i + 1], staples[4 * i + 2], staples[4 * i + 3]};
-    Q<double> U, U0 = qzero<double>();
-    double r = heatbath_link64(A, beta, src, U0, U);
+    Q<double> U;
+    int c = heatbath_link64(A, beta, src, U);
     links[4 * i] = U.c0; links[4 * i + 1] = U.c1; links[4 * i + 2] = U.c2; links[4 * i + 3] = U.c3;
-    counts[i] = (r > 0.0) ? (int)(1.0 / r + 0.5) : 0;
-    if (src.pos > src.n) counts[i] = -1;
+    counts[i] = (src.pos > src.n) ? -1 : c;
 }
 
 static __global__ void __launch_bounds__(BLOCK) k_or_harness(int n, const double* __restrict__ staples,
