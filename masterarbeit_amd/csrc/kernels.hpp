@@ -30,10 +30,13 @@ constexpr double PI_D = 3.14159265358979323846;   // = 2 asin(1), LGT_Code/modul
 //         away in the same time slice) and the +-t neighbours (g.gs tiles
 //         away) both stay close in the sequence.
 // Returns the first cb index of the block's tile.
-__device__ __forceinline__ uint32_t tile_base(const Geo& g)
+__device__ __forceinline__ uint32_t xcd_remap(const Geo& g, uint32_t b, uint32_t nb)
 {
-    uint32_t b = blockIdx.x, nb = gridDim.x;
-    if ((g.order & 1) && (nb & 7u) == 0) b = (b & 7u) * (nb >> 3) + (b >> 3);
+    return ((g.order & 1) && (nb & 7u) == 0) ? (b & 7u) * (nb >> 3) + (b >> 3) : b;
+}
+
+__device__ __forceinline__ uint32_t tile_map(const Geo& g, uint32_t b)
+{
     if (g.order & 4) {
         uint32_t grp = fdiv(g.dgt, b);
         uint32_t r = b - grp * ((uint32_t)g.gs * (uint32_t)g.L[0]);
@@ -47,6 +50,23 @@ __device__ __forceinline__ uint32_t tile_base(const Geo& g)
         return t * (g.vh / (uint32_t)g.L[0]) + s * 256u;
     }
     return b * 256u;
+}
+
+__device__ __forceinline__ uint32_t tile_base(const Geo& g)
+{
+    return tile_map(g, xcd_remap(g, blockIdx.x, gridDim.x));
+}
+
+// Launches over both parities (grid x = 2 * tiles): the two parities of a
+// tile are consecutive blocks of the (remapped) sequence, so they run side by
+// side on one XCD and read the neighbours they share (the other parity's
+// links of the same few rows) through one L2 instead of in two passes over
+// the field.  Sets p and returns the first cb index of the tile.
+__device__ __forceinline__ uint32_t tile_base_pair(const Geo& g, int& p)
+{
+    uint32_t b = xcd_remap(g, blockIdx.x, gridDim.x);
+    p = (int)(b & 1u);
+    return tile_map(g, b >> 1);
 }
 
 // ---------------------------------------------------------------------------
@@ -453,14 +473,14 @@ __device__ __forceinline__ T plaq_term(const Planes<T>& P, const Geo& g, const S
     return T(2) * un0.c0;
 }
 
-// grid: x over cb sites, y = parity; partial[y*gridDim.x + tile]
+// grid: x over (tile, parity) pairs; partial[p*tiles + tile]
 template <typename T>
 __global__ void __launch_bounds__(BLOCK) k_plaquette(Planes<T> P, Geo g, DD* __restrict__ partial)
 {
-    uint32_t base = tile_base(g);
+    int p;
+    uint32_t base = tile_base_pair(g, p);
     uint32_t tile = base / BLOCK;
     uint32_t i = base + threadIdx.x;
-    int p = blockIdx.y;
     DD acc = DD{0.0, 0.0};
     if (i < g.vh) {
         Site s = cb_site(g, i, p);
@@ -470,7 +490,7 @@ __global__ void __launch_bounds__(BLOCK) k_plaquette(Planes<T> P, Geo g, DD* __r
             for (int kk = jj + 1; kk < 4; kk++) acc = dd_add(acc, (double)plaq_term(P, g, s, jj, kk));
     }
     acc = block_sum_dd(acc);
-    if (threadIdx.x == 0) partial[blockIdx.y * gridDim.x + tile] = acc;
+    if (threadIdx.x == 0) partial[p * (gridDim.x >> 1) + tile] = acc;
 }
 
 // ---------------------------------------------------------------------------
@@ -506,7 +526,7 @@ __device__ __forceinline__ Q<T> ape_staple(const Planes<T>& Z, const Geo& g, con
 // Z_a(n-a+b)) are loaded once: 24 loads for the 3 spatial links instead of
 // 36, 44 instead of 72 for all four.  Each staple is still summed in the
 // reference's order (k ascending, forward then backward term).
-// grid: x over cb sites, y = parity.  With Mn.p[1][0] set (spatial mode),
+// grid: x over (tile, parity) pairs.  With Mn.p[1][0] set (spatial mode),
 // |M| = |Z + sp S| of every smeared link is stored too (for the adjoint).
 template <typename T> struct NormPlanes { T* p[4][2]; };
 
@@ -514,9 +534,9 @@ template <typename T, int MODE>
 __global__ void __launch_bounds__(BLOCK) k_ape(Planes<T> Z, Planes<T> O, Geo g, T sp, NormPlanes<T> Mn)
 {
     constexpr int MU0 = (MODE == 0) ? 1 : 0, MU1 = (MODE == 2) ? 0 : 3;
-    uint32_t i = tile_base(g) + threadIdx.x;
+    int p;
+    uint32_t i = tile_base_pair(g, p) + threadIdx.x;
     if (i >= g.vh) return;
-    const int p = blockIdx.y;
     Site s = cb_site(g, i, p);
     // directions that enter the staples
     constexpr int K0 = (MODE == 1) ? 0 : 1;
@@ -593,12 +613,12 @@ __global__ void __launch_bounds__(BLOCK) k_wloops(Planes<T> P, Geo g, const WGro
                                                   int ncols)
 {
     __shared__ DD wacc[NWAVES][WMAXT];
-    uint32_t base = tile_base(g);
+    int p;
+    uint32_t base = tile_base_pair(g, p);
     uint32_t tile = base / BLOCK;
     uint32_t i = base + threadIdx.x;
-    const int p = blockIdx.y;
-    const WGroup grp = groups[blockIdx.z / 3];
-    const int d = 1 + (int)(blockIdx.z % 3);
+    const WGroup grp = groups[blockIdx.y / 3];
+    const int d = 1 + (int)(blockIdx.y % 3);
     const int R = grp.R;
     const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
     for (int t = threadIdx.x; t < NWAVES * WMAXT; t += BLOCK) wacc[t / WMAXT][t % WMAXT] = DD{0.0, 0.0};
@@ -632,7 +652,7 @@ __global__ void __launch_bounds__(BLOCK) k_wloops(Planes<T> P, Geo g, const WGro
         if (slot < 0) continue;
         DD a = wacc[0][t];
         for (int w = 1; w < NWAVES; w++) a = dd_add(a, wacc[w][t]);
-        int col = ((d - 1) * 2 + p) * gridDim.x + tile;
+        int col = ((d - 1) * 2 + p) * (gridDim.x >> 1) + tile;
         partial[(size_t)slot * ncols + col] = a;
     }
 }
@@ -798,16 +818,17 @@ __device__ __forceinline__ T clover_products_1x1(const Planes<T>& P, const Geo& 
 
 // KIND 0: improved (8 cp(0123) + 8 cp(0231) + 8 cp(0312)),
 // KIND 1: 1x1 clover (-cp(0132) - cp(1230) - cp(2031)).
-// partial[k] = sum q and partial[ncols + k] = sum |q| with k = y*gridDim.x +
-// tile, ncols = 2*gridDim.x; qx: density in lexicographic order (nullable).
+// grid: x over (tile, parity) pairs.  partial[k] = sum q and
+// partial[ncols + k] = sum |q| with k = p*tiles + tile, ncols = 2*tiles;
+// qx: density in lexicographic order (nullable).
 template <typename T, int KIND>
 __global__ void __launch_bounds__(BLOCK) k_qtop(Planes<T> P, Geo g, DD* __restrict__ partial,
                                                 double* __restrict__ qx)
 {
-    uint32_t base = tile_base(g);
+    int p;
+    uint32_t base = tile_base_pair(g, p);
     uint32_t tile = base / BLOCK;
     uint32_t i = base + threadIdx.x;
-    int p = blockIdx.y;
     double q = 0.0;
     if (i < g.vh) {
         Site s = cb_site(g, i, p);
@@ -824,9 +845,9 @@ __global__ void __launch_bounds__(BLOCK) k_qtop(Planes<T> P, Geo g, DD* __restri
     DD a = block_sum_dd(DD{q, 0.0});
     DD b = block_sum_dd(DD{fabs(q), 0.0});
     if (threadIdx.x == 0) {
-        size_t k = (size_t)blockIdx.y * gridDim.x + tile;
+        size_t k = (size_t)p * (gridDim.x >> 1) + tile;
         partial[k] = a;
-        partial[2 * (size_t)gridDim.x + k] = b;
+        partial[(size_t)gridDim.x + k] = b;
     }
 }
 

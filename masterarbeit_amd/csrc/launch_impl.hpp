@@ -84,14 +84,14 @@ hipError_t Launch<T>::overrelax(hipStream_t s, const Geo& g, const RawPlanes& P,
 template <typename T>
 hipError_t Launch<T>::plaquette(hipStream_t s, const Geo& g, const RawPlanes& P, void* partial)
 {
-    hipLaunchKernelGGL(k_plaquette<T>, dim3(grid_x(g), 2), dim3(BLOCK), 0, s, planes<T>(P), g, (DD*)partial);
+    hipLaunchKernelGGL(k_plaquette<T>, dim3(2 * grid_x(g)), dim3(BLOCK), 0, s, planes<T>(P), g, (DD*)partial);
     return hipGetLastError();
 }
 
 template <typename T>
 hipError_t Launch<T>::ape(hipStream_t s, const Geo& g, const RawPlanes& in, const RawPlanes& out, int mode, double sp)
 {
-    dim3 grid(grid_x(g), 2), block(BLOCK);
+    dim3 grid(2 * grid_x(g)), block(BLOCK);
     Planes<T> Z = planes<T>(in), O = planes<T>(out);
     NormPlanes<T> none{};
     if (mode == 0) hipLaunchKernelGGL((k_ape<T, 0>), grid, block, 0, s, Z, O, g, (T)sp, none);
@@ -104,7 +104,7 @@ template <typename T>
 hipError_t Launch<T>::wloops(hipStream_t s, const Geo& g, const RawPlanes& P, const WGroupHost* groups, int ngroups,
                              const int* slots, void* partial, int ncols)
 {
-    dim3 grid(grid_x(g), 2, 3 * ngroups), block(BLOCK);
+    dim3 grid(2 * grid_x(g), 3 * ngroups), block(BLOCK);
     hipLaunchKernelGGL(k_wloops<T>, grid, block, 0, s, planes<T>(P), g, (const WGroup*)groups, slots,
                        (DD*)partial, ncols);
     return hipGetLastError();
@@ -114,7 +114,7 @@ template <typename T>
 hipError_t Launch<T>::qtop(hipStream_t s, const Geo& g, const RawPlanes& P, int kind, void* partial,
                            double* density)
 {
-    dim3 grid(grid_x(g), 2), block(BLOCK);
+    dim3 grid(2 * grid_x(g)), block(BLOCK);
     if (kind == 0) hipLaunchKernelGGL((k_qtop<T, 0>), grid, block, 0, s, planes<T>(P), g, (DD*)partial, density);
     else hipLaunchKernelGGL((k_qtop<T, 1>), grid, block, 0, s, planes<T>(P), g, (DD*)partial, density);
     return hipGetLastError();
@@ -133,7 +133,7 @@ template <typename T>
 hipError_t Launch<T>::ape_save(hipStream_t s, const Geo& g, const RawPlanes& in, const RawPlanes& out,
                                const RawNorms& mn, double sp)
 {
-    hipLaunchKernelGGL((k_ape<T, 0>), dim3(grid_x(g), 2), dim3(BLOCK), 0, s, planes<T>(in), planes<T>(out), g, (T)sp,
+    hipLaunchKernelGGL((k_ape<T, 0>), dim3(2 * grid_x(g)), dim3(BLOCK), 0, s, planes<T>(in), planes<T>(out), g, (T)sp,
                        norms<T>(mn));
     return hipGetLastError();
 }
@@ -152,7 +152,7 @@ hipError_t Launch<T>::vjp_step(hipStream_t s, const Geo& g, const RawPlanes& Z, 
                                const RawNorms* mprev, const RawPlanes& out, double sp)
 {
     RawNorms none{};
-    hipLaunchKernelGGL(k_vjp_step<T>, dim3(grid_x(g), 2), dim3(BLOCK), 0, s, planes<T>(Z), planes<T>(Hin),
+    hipLaunchKernelGGL(k_vjp_step<T>, dim3(2 * grid_x(g)), dim3(BLOCK), 0, s, planes<T>(Z), planes<T>(Hin),
                        norms<T>(mprev ? *mprev : none), mprev ? 1 : 0, planes<T>(out), g, (T)sp);
     return hipGetLastError();
 }
@@ -160,7 +160,7 @@ hipError_t Launch<T>::vjp_step(hipStream_t s, const Geo& g, const RawPlanes& Z, 
 template <typename T>
 hipError_t Launch<T>::wloop_grad(hipStream_t s, const Geo& g, const RawPlanes& U, int R, int T_, const RawPlanes& G)
 {
-    hipLaunchKernelGGL(k_wloop_grad<T>, dim3(grid_x(g), 8), dim3(BLOCK), 0, s, planes<T>(U), g, R, T_, planes<T>(G));
+    hipLaunchKernelGGL(k_wloop_grad<T>, dim3(2 * grid_x(g), 4), dim3(BLOCK), 0, s, planes<T>(U), g, R, T_, planes<T>(G));
     return hipGetLastError();
 }
 

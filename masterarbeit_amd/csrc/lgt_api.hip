@@ -239,13 +239,31 @@ int alloc_field(lgt_engine* e, void** base, RawPlanes* P)
     return LGT_OK;
 }
 
-int alloc_norms(lgt_engine* e, void** base, RawNorms* N)
+// Buffers of the smearing adjoint hold the spatial links only (the temporal
+// links neither change under spatial APE nor enter its staples): planes
+// [mu][p] for mu = 1..3 at ((mu-1)*2 + p) * VOL/2, p[0][*] null.
+size_t spatial_field_bytes(const lgt_engine* e) { return (size_t)3 * e->vol * e->link_bytes; }
+size_t spatial_norm_bytes(const lgt_engine* e) { return (size_t)3 * e->vol * (e->link_bytes / 4); }
+
+int alloc_spatial_field(lgt_engine* e, void** base, RawPlanes* P)
 {
-    size_t el = e->link_bytes / 4, bytes = (size_t)4 * e->vol * el, have = 0;
+    size_t have = 0;
     *base = nullptr;
-    TRY(ensure(e, base, &have, bytes, "adjoint norms"));
-    for (int mu = 0; mu < 4; mu++)
-        for (int p = 0; p < 2; p++) N->p[mu][p] = (char*)*base + ((size_t)(mu * 2 + p) * e->geo.vh) * el;
+    TRY(ensure(e, base, &have, spatial_field_bytes(e), "adjoint field"));
+    for (int p = 0; p < 2; p++) P->p[0][p] = nullptr;
+    for (int mu = 1; mu < 4; mu++)
+        for (int p = 0; p < 2; p++) P->p[mu][p] = (char*)*base + ((size_t)((mu - 1) * 2 + p) * e->geo.vh) * e->link_bytes;
+    return LGT_OK;
+}
+
+int alloc_spatial_norms(lgt_engine* e, void** base, RawNorms* N)
+{
+    size_t el = e->link_bytes / 4, have = 0;
+    *base = nullptr;
+    TRY(ensure(e, base, &have, spatial_norm_bytes(e), "adjoint norms"));
+    for (int p = 0; p < 2; p++) N->p[0][p] = nullptr;
+    for (int mu = 1; mu < 4; mu++)
+        for (int p = 0; p < 2; p++) N->p[mu][p] = (char*)*base + ((size_t)((mu - 1) * 2 + p) * e->geo.vh) * el;
     return LGT_OK;
 }
 
@@ -261,7 +279,7 @@ void free_buf(lgt_engine* e, void*& b, size_t bytes)
 void free_vjp(lgt_engine* e, bool keep_cot)
 {
     VjpState& v = e->vjp;
-    size_t fb = (size_t)4 * e->vol * e->link_bytes, nb = (size_t)4 * e->vol * (e->link_bytes / 4);
+    size_t fb = spatial_field_bytes(e), nb = spatial_norm_bytes(e);
     for (auto& b : v.ck) free_buf(e, b, fb);
     for (auto& b : v.segU) free_buf(e, b, fb);
     for (auto& b : v.segM) free_buf(e, b, nb);
@@ -270,10 +288,11 @@ void free_vjp(lgt_engine* e, bool keep_cot)
     free_buf(e, v.g[0], fb);
     free_buf(e, v.g[1], fb);
     if (!keep_cot) {
-        free_buf(e, v.gobj, fb);
+        free_buf(e, v.gobj, (size_t)4 * e->vol * e->link_bytes);
         v.cot = false;
     }
     v.fwd = false;
+    v.nsteps = v.seg = v.nck = 0;
 }
 
 int copy_spatial(lgt_engine* e, const RawPlanes& dst, const RawPlanes& src)
@@ -677,22 +696,39 @@ int lgt_ape_vjp_forward(lgt_engine* e, int nsteps, double sp, int segment)
     if (!e || nsteps < 1 || segment < 0) return fail(LGT_EINVAL, "lgt_ape_vjp_forward: nsteps %d segment %d", nsteps, segment);
     TRY(set_dev(e));
     VjpState& v = e->vjp;
-    free_vjp(e, false);
     int seg = segment > 0 ? segment : (int)std::ceil(std::sqrt((double)nsteps));
     if (seg > nsteps) seg = nsteps;
-    v.nsteps = nsteps;
-    v.seg = seg;
-    v.nck = (nsteps + seg - 1) / seg;
-    v.sp = sp;
-    v.ck.assign(v.nck, nullptr);
-    v.ckP.resize(v.nck);
-    for (int c = 0; c < v.nck; c++) {
-        int rc = alloc_field(e, &v.ck[c], &v.ckP[c]);
+    const int nck = (nsteps + seg - 1) / seg;
+    // checkpoints, segment fields and norms, cotangents: allocated here once
+    // for a layout (seg, nck) and kept for later calls with the same layout,
+    // so neither pass allocates when the adjoint is repeated
+    if (v.seg != seg || v.nck != nck) {
+        free_vjp(e, true);
+        v.ck.assign(nck, nullptr);
+        v.ckP.resize(nck);
+        v.segU.assign(seg, nullptr);
+        v.segUP.resize(seg);
+        v.segM.assign(seg, nullptr);
+        v.segMN.resize(seg);
+        int rc = LGT_OK;
+        for (int c = 0; c < nck && rc == LGT_OK; c++) rc = alloc_spatial_field(e, &v.ck[c], &v.ckP[c]);
+        for (int k = 0; k < seg && rc == LGT_OK; k++) {
+            rc = alloc_spatial_field(e, &v.segU[k], &v.segUP[k]);
+            if (rc == LGT_OK) rc = alloc_spatial_norms(e, &v.segM[k], &v.segMN[k]);
+        }
+        if (rc == LGT_OK) rc = alloc_spatial_field(e, &v.g[0], &v.gP[0]);
+        if (rc == LGT_OK) rc = alloc_spatial_field(e, &v.g[1], &v.gP[1]);
         if (rc != LGT_OK) {
-            free_vjp(e, false);
+            free_vjp(e, true);
             return rc;
         }
+        v.seg = seg;
+        v.nck = nck;
     }
+    v.nsteps = nsteps;
+    v.sp = sp;
+    v.fwd = false;
+    v.cot = false;
     if (!e->field_alt) {
         size_t have = 0;
         TRY(ensure(e, &e->field_alt, &have, (size_t)4 * e->vol * e->link_bytes, "APE buffer"));
@@ -771,18 +807,6 @@ int lgt_ape_vjp_backward(lgt_engine* e, const double* cotangent, double* grad)
         v.cot = true;
     }
     if (!v.cot) return fail(LGT_EINVAL, "lgt_ape_vjp_backward: no cotangent (pass one or call lgt_wloop_grad)");
-    if (v.segU.empty()) {
-        v.segU.assign(v.seg, nullptr);
-        v.segUP.resize(v.seg);
-        v.segM.assign(v.seg, nullptr);
-        v.segMN.resize(v.seg);
-        for (int k = 0; k < v.seg; k++) {
-            TRY(alloc_field(e, &v.segU[k], &v.segUP[k]));
-            TRY(alloc_norms(e, &v.segM[k], &v.segMN[k]));
-        }
-        TRY(alloc_field(e, &v.g[0], &v.gP[0]));
-        TRY(alloc_field(e, &v.g[1], &v.gP[1]));
-    }
     int h = 0;   // gP[h]: the running cotangent
     const double sp = v.sp;
     for (int s = v.nck - 1; s >= 0; s--) {

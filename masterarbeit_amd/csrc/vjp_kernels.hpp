@@ -50,15 +50,15 @@ __device__ __forceinline__ Q<T> mul3(const Q<T>& a, const Q<T>& b, const Q<T>& c
 // that earlier step: (G - <G,Z> Z)/Mprev  (Z is that step's output).
 // One thread handles the three spatial links of its site, so the gathers the
 // directions share are loaded once (51 instead of 81).
-// grid: x over cb sites, y = parity
+// grid: x over (tile, parity) pairs
 template <typename T>
 __global__ void __launch_bounds__(BLOCK) k_vjp_step(Planes<T> Z, Planes<T> Hin, NormPlanes<T> Mprev, int has_prev,
                                                     Planes<T> Out, Geo g, T sp)
 {
 #pragma clang fp contract(fast)
-    uint32_t i = tile_base(g) + threadIdx.x;
+    int p;
+    uint32_t i = tile_base_pair(g, p) + threadIdx.x;
     if (i >= g.vh) return;
-    const int p = blockIdx.y;
     Site y = cb_site(g, i, p);
     Site yp[4], ym[4];
 #pragma unroll
@@ -73,17 +73,27 @@ __global__ void __launch_bounds__(BLOCK) k_vjp_step(Planes<T> Z, Planes<T> Hin, 
 #pragma unroll
         for (int b = 1; b < 4; b++) {
             if (b == a) continue;
-            Site ymba = shift(g, ym[b], a, 1);
-            Q<T> Za_yb = link(Z, g, a, yp[b]), Zb_ya = link(Z, g, b, yp[a]), Zb_y = link(Z, g, b, y);
-            Q<T> Zb_ymb = link(Z, g, b, ym[b]), Za_ymb = link(Z, g, a, ym[b]), Zb_ymba = link(Z, g, b, ymba);
-            Q<T> Hb_y = link(Hin, g, b, y), Ha_ymb = link(Hin, g, a, ym[b]), Hb_ymb = link(Hin, g, b, ym[b]);
-            Q<T> Hb_ya = link(Hin, g, b, yp[a]), Ha_yb = link(Hin, g, a, yp[b]), Hb_ymba = link(Hin, g, b, ymba);
-            acc = qadd(acc, mul3(Hb_y, Za_yb, qdag(Zb_ya)));            // t1
-            acc = qadd(acc, mul3(qdag(Zb_ymb), Ha_ymb, Zb_ymba));       // t2
-            acc = qadd(acc, mul3(qdag(Hb_ymb), Za_ymb, Zb_ymba));       // t3
-            acc = qadd(acc, mul3(Zb_y, Za_yb, qdag(Hb_ya)));            // t4
-            acc = qadd(acc, mul3(Zb_y, Ha_yb, qdag(Zb_ya)));            // t5
-            acc = qadd(acc, mul3(qdag(Zb_ymb), Za_ymb, Hb_ymba));       // t6
+            // forward staple: t1 + t5 = (H_b(y) Z_a(y+b) + Z_b(y) H_a(y+b)) Z_b(y+a)^+,
+            //                 t4 = Z_b(y) Z_a(y+b) H_b(y+a)^+   (5 products, not 6)
+            {
+                Q<T> Zb_y = link(Z, g, b, y), Hb_y = link(Hin, g, b, y);
+                Q<T> Za_yb = link(Z, g, a, yp[b]), Ha_yb = link(Hin, g, a, yp[b]);
+                Q<T> up = qadd(qmul(Hb_y, Za_yb), qmul(Zb_y, Ha_yb));
+                Q<T> zz = qmul(Zb_y, Za_yb);
+                acc = qadd(acc, qmul_dag(up, link(Z, g, b, yp[a])));
+                acc = qadd(acc, qmul_dag(zz, link(Hin, g, b, yp[a])));
+            }
+            // backward staple: t2 + t3 = (Z_b(y-b)^+ H_a(y-b) + H_b(y-b)^+ Z_a(y-b)) Z_b(y-b+a),
+            //                  t6 = Z_b(y-b)^+ Z_a(y-b) H_b(y-b+a)
+            {
+                Site ymba = shift(g, ym[b], a, 1);
+                Q<T> Zb_ymb = link(Z, g, b, ym[b]), Hb_ymb = link(Hin, g, b, ym[b]);
+                Q<T> Za_ymb = link(Z, g, a, ym[b]), Ha_ymb = link(Hin, g, a, ym[b]);
+                Q<T> lo = qadd(qmul(qdag(Zb_ymb), Ha_ymb), qmul(qdag(Hb_ymb), Za_ymb));
+                Q<T> zz = qmul(qdag(Zb_ymb), Za_ymb);
+                acc = qadd(acc, qmul(lo, link(Z, g, b, ymba)));
+                acc = qadd(acc, qmul(zz, link(Hin, g, b, ymba)));
+            }
         }
         Q<T> G = qadd(link(Hin, g, a, y), qscale(acc, sp));
         if (has_prev) {
@@ -118,14 +128,15 @@ __device__ __forceinline__ void seg(const Planes<T>& U, const Geo& g, Q<T>& acc,
     }
 }
 
-// grid: x over cb sites, y = mu*2 + parity
+// grid: x over (tile, parity) pairs, y = mu
 template <typename T>
 __global__ void __launch_bounds__(BLOCK) k_wloop_grad(Planes<T> U, Geo g, int R, int Tt, Planes<T> G)
 {
 #pragma clang fp contract(fast)
-    uint32_t i = tile_base(g) + threadIdx.x;
+    int p;
+    uint32_t i = tile_base_pair(g, p) + threadIdx.x;
     if (i >= g.vh) return;
-    int mu = blockIdx.y >> 1, p = blockIdx.y & 1;
+    int mu = blockIdx.y;
     Site y = cb_site(g, i, p);
     Q<T> grad = qzero<T>();
     if (mu == 0) {
