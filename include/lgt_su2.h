@@ -56,6 +56,9 @@ typedef struct lgt_engine lgt_engine;
                                 groups of spatial tiles (2 and 4 apply when a time slice
                                 holds whole tiles).  Results do not depend on it. */
 #define LGT_OPT_PROFILE 1    /* 1: time every launch with HIP events */
+#define LGT_OPT_GROUP 2      /* spatial tiles per group of the grouped tile order: 0 (default)
+                                = tuned for the lattice; else a divisor of the tiles in one
+                                time slice.  Results do not depend on it. */
 
 /* kernel classes of the launch profile (lgt_profile_read) */
 #define LGT_K_HEATBATH 0

@@ -53,9 +53,10 @@ struct VjpState {
     int nsteps = 0, seg = 0, nck = 0;
     double sp = 0.0;
     bool fwd = false, cot = false;
-    std::vector<void*> ck, segU, segM;
+    bool top_saved = false;   // segment buffers hold the forward's last segment
+    std::vector<void*> ck, ckM, segU, segM;
     std::vector<RawPlanes> ckP, segUP;
-    std::vector<RawNorms> segMN;
+    std::vector<RawNorms> ckMN, segMN;   // ckMN[c]: |M| of the step before checkpoint c (c >= 1)
     void* g[2] = {nullptr, nullptr};
     RawPlanes gP[2];
     void* gobj = nullptr;
@@ -186,6 +187,35 @@ int flush_profile(lgt_engine* e)
 
 uint32_t nbx(const lgt_engine* e) { return grid_x(e->geo); }
 
+// Group size of the grouped tile order.  Auto (0): the largest divisor of the
+// tiles of one time slice that is at most an eighth of them, so that with the
+// XCD-contiguous order each of the 8 XCDs walks its own spatial slab through
+// all time slices (measured best on MI355X at 48^4, 27 tiles, and at 96^4, 216
+// tiles).  Returns false for a size that does not divide the tiles of a time
+// slice.
+bool set_group(Geo& g, int gs)
+{
+    if ((g.vh / (uint32_t)g.L[0]) % 256u != 0) {
+        g.gs = 0;
+        return gs == 0;
+    }
+    uint32_t tiles = g.vh / (uint32_t)g.L[0] / 256u;
+    if (gs == 0) {
+        uint32_t cap = std::max(1u, tiles / 8u);
+        for (uint32_t d = std::min(cap, tiles); d >= 1; d--)
+            if (tiles % d == 0) {
+                gs = (int)d;
+                break;
+            }
+    } else if (gs < 0 || tiles % (uint32_t)gs != 0) {
+        return false;
+    }
+    g.gs = gs;
+    g.dgs = make_fastdiv((uint32_t)g.gs);
+    g.dgt = make_fastdiv((uint32_t)g.gs * (uint32_t)g.L[0]);
+    return true;
+}
+
 // the time-fastest and grouped orders need whole 256-site tiles per time slice
 int tile_order_ok(const Geo& g, int order)
 {
@@ -194,16 +224,16 @@ int tile_order_ok(const Geo& g, int order)
     return order;
 }
 
-// Geometry of one launch with its tile order.  Measured on MI355X at 48^4:
-// the grouped, XCD-contiguous order (5) serves the staple gathers of every
-// direction best; without whole tiles per time slice the XCD-contiguous
-// linear order (1) is used.
+// Geometry of one launch with its tile order.  Measured on MI355X at 48^4 and
+// 96^4: the grouped, XCD-contiguous order (5) serves the staple gathers of
+// every kernel best, except the loop-gradient kernel, whose long temporal
+// walks prefer plain time-fastest tiles (3; -20..30 %); without whole tiles
+// per time slice the XCD-contiguous linear order (1) is used.
 Geo kgeo(const lgt_engine* e, int cls, int mu)
 {
-    (void)cls;
     (void)mu;
     Geo g = e->geo;
-    int order = (e->order >= 0) ? e->order : (g.gs ? 5 : 1);
+    int order = (e->order >= 0) ? e->order : (cls == LGT_K_WLOOP_GRAD ? 3 : (g.gs ? 5 : 1));
     g.order = tile_order_ok(g, order);
     return g;
 }
@@ -281,10 +311,12 @@ void free_vjp(lgt_engine* e, bool keep_cot)
     VjpState& v = e->vjp;
     size_t fb = spatial_field_bytes(e), nb = spatial_norm_bytes(e);
     for (auto& b : v.ck) free_buf(e, b, fb);
+    for (auto& b : v.ckM) free_buf(e, b, nb);
     for (auto& b : v.segU) free_buf(e, b, fb);
     for (auto& b : v.segM) free_buf(e, b, nb);
-    v.ck.clear(); v.segU.clear(); v.segM.clear();
-    v.ckP.clear(); v.segUP.clear(); v.segMN.clear();
+    v.ck.clear(); v.ckM.clear(); v.segU.clear(); v.segM.clear();
+    v.ckP.clear(); v.ckMN.clear(); v.segUP.clear(); v.segMN.clear();
+    v.top_saved = false;
     free_buf(e, v.g[0], fb);
     free_buf(e, v.g[1], fb);
     if (!keep_cot) {
@@ -370,18 +402,8 @@ int lgt_create(const int extents[4], int precision, uint64_t seed, lgt_engine** 
     g.dx = make_fastdiv((uint32_t)extents[1]);
     g.dt = make_fastdiv((uint32_t)extents[0]);
     g.order = 0;
-    // grouped order: the largest divisor <= 27 of the tiles in one time slice
     g.gs = 0;
-    if ((g.vh / (uint32_t)extents[0]) % 256u == 0) {
-        uint32_t tiles = g.vh / (uint32_t)extents[0] / 256u;
-        for (uint32_t d = 27; d >= 1; d--)
-            if (tiles % d == 0) {
-                g.gs = (int)d;
-                break;
-            }
-        g.dgs = make_fastdiv((uint32_t)g.gs);
-        g.dgt = make_fastdiv((uint32_t)g.gs * (uint32_t)extents[0]);
-    }
+    set_group(g, 0);
     e->link_bytes = (precision == LGT_FP64) ? 32 : 16;
 
     err = hipStreamCreateWithFlags(&e->stream, hipStreamNonBlocking);
@@ -435,6 +457,9 @@ int lgt_set_option(lgt_engine* e, int option, int value)
     case LGT_OPT_TILE_ORDER:
         if (value < -1 || value > 7) return fail(LGT_EINVAL, "lgt_set_option: tile order %d", value);
         e->order = value;
+        return LGT_OK;
+    case LGT_OPT_GROUP:
+        if (!set_group(e->geo, value)) return fail(LGT_EINVAL, "lgt_set_option: group size %d", value);
         return LGT_OK;
     case LGT_OPT_PROFILE:
         if (!value) TRY(flush_profile(e));
@@ -706,18 +731,21 @@ int lgt_ape_vjp_forward(lgt_engine* e, int nsteps, double sp, int segment)
         free_vjp(e, true);
         v.ck.assign(nck, nullptr);
         v.ckP.resize(nck);
+        v.ckM.assign(nck, nullptr);
+        v.ckMN.resize(nck);
+        int rc = LGT_OK;
+        for (int c = 0; c < nck && rc == LGT_OK; c++) rc = alloc_spatial_field(e, &v.ck[c], &v.ckP[c]);
+        for (int c = 1; c < nck && rc == LGT_OK; c++) rc = alloc_spatial_norms(e, &v.ckM[c], &v.ckMN[c]);
+        if (rc == LGT_OK) rc = alloc_spatial_field(e, &v.g[0], &v.gP[0]);
+        if (rc == LGT_OK) rc = alloc_spatial_field(e, &v.g[1], &v.gP[1]);
         v.segU.assign(seg, nullptr);
         v.segUP.resize(seg);
         v.segM.assign(seg, nullptr);
         v.segMN.resize(seg);
-        int rc = LGT_OK;
-        for (int c = 0; c < nck && rc == LGT_OK; c++) rc = alloc_spatial_field(e, &v.ck[c], &v.ckP[c]);
-        for (int k = 0; k < seg && rc == LGT_OK; k++) {
-            rc = alloc_spatial_field(e, &v.segU[k], &v.segUP[k]);
-            if (rc == LGT_OK) rc = alloc_spatial_norms(e, &v.segM[k], &v.segMN[k]);
+        for (int j = 0; j < seg && rc == LGT_OK; j++) {
+            rc = alloc_spatial_field(e, &v.segU[j], &v.segUP[j]);
+            if (rc == LGT_OK) rc = alloc_spatial_norms(e, &v.segM[j], &v.segMN[j]);
         }
-        if (rc == LGT_OK) rc = alloc_spatial_field(e, &v.g[0], &v.gP[0]);
-        if (rc == LGT_OK) rc = alloc_spatial_field(e, &v.g[1], &v.gP[1]);
         if (rc != LGT_OK) {
             free_vjp(e, true);
             return rc;
@@ -734,12 +762,35 @@ int lgt_ape_vjp_forward(lgt_engine* e, int nsteps, double sp, int segment)
         TRY(ensure(e, &e->field_alt, &have, (size_t)4 * e->vol * e->link_bytes, "APE buffer"));
         set_planes(e->alt, e->field_alt, e->geo.vh, e->link_bytes);
     }
+    // Steps before the last segment smear cur -> alt (and swap); the step just
+    // before a checkpoint also keeps its |M| (the backward then emits the
+    // tangent cotangent across the segment boundary itself).  The last
+    // segment's steps keep their outputs and |M| in the segment buffers, so
+    // the backward starts without recomputing it; its final step writes alt.
+    const int n0_last = (nck - 1) * seg;
+    v.top_saved = false;
     for (int n = 0; n < nsteps; n++) {
         if (n % seg == 0) TRY(copy_spatial(e, v.ckP[n / seg], e->cur));
-        TRY(run(e, LGT_K_APE, "ape", [&] { return DISPATCH(e, ape(e->stream, kgeo(e, LGT_K_APE, 0), e->cur, e->alt, 0, sp)); }));
-        for (int mu = 1; mu < 4; mu++)
-            for (int p = 0; p < 2; p++) std::swap(e->cur.p[mu][p], e->alt.p[mu][p]);
+        if (n < n0_last) {
+            if ((n + 1) % seg == 0) {
+                TRY(run(e, LGT_K_APE, "ape", [&] {
+                    return DISPATCH(e, ape_save(e->stream, kgeo(e, LGT_K_APE, 0), e->cur, e->alt, v.ckMN[(n + 1) / seg], sp));
+                }));
+            } else {
+                TRY(run(e, LGT_K_APE, "ape", [&] { return DISPATCH(e, ape(e->stream, kgeo(e, LGT_K_APE, 0), e->cur, e->alt, 0, sp)); }));
+            }
+        } else {
+            const RawPlanes& in = (n == n0_last) ? e->cur : v.segUP[n - n0_last - 1];
+            const RawPlanes& out = (n == nsteps - 1) ? e->alt : v.segUP[n - n0_last];
+            TRY(run(e, LGT_K_APE, "ape", [&] {
+                return DISPATCH(e, ape_save(e->stream, kgeo(e, LGT_K_APE, 0), in, out, v.segMN[n - n0_last], sp));
+            }));
+        }
+        if (n < n0_last || n == nsteps - 1)
+            for (int mu = 1; mu < 4; mu++)
+                for (int p = 0; p < 2; p++) std::swap(e->cur.p[mu][p], e->alt.p[mu][p]);
     }
+    v.top_saved = true;
     v.fwd = true;
     HIP_TRY(hipStreamSynchronize(e->stream), "hipStreamSynchronize");
     return LGT_OK;
@@ -811,22 +862,24 @@ int lgt_ape_vjp_backward(lgt_engine* e, const double* cotangent, double* grad)
     const double sp = v.sp;
     for (int s = v.nck - 1; s >= 0; s--) {
         const int n0 = s * v.seg, n1 = std::min(n0 + v.seg, v.nsteps);
-        for (int n = n0; n < n1; n++) {
-            const RawPlanes& in = (n == n0) ? v.ckP[s] : v.segUP[n - n0 - 1];
-            TRY(run(e, LGT_K_APE_SAVE, "ape recompute", [&] {
-                return DISPATCH(e, ape_save(e->stream, kgeo(e, LGT_K_APE_SAVE, 0), in, v.segUP[n - n0], v.segMN[n - n0], sp));
+        if (!(s == v.nck - 1 && v.top_saved)) {
+            v.top_saved = false;
+            for (int n = n0; n < n1; n++) {
+                const RawPlanes& in = (n == n0) ? v.ckP[s] : v.segUP[n - n0 - 1];
+                TRY(run(e, LGT_K_APE_SAVE, "ape recompute", [&] {
+                    return DISPATCH(e, ape_save(e->stream, kgeo(e, LGT_K_APE_SAVE, 0), in, v.segUP[n - n0], v.segMN[n - n0], sp));
+                }));
+            }
+        }
+        if (s == v.nck - 1) {
+            TRY(run(e, LGT_K_VJP, "adjoint tangent", [&] {
+                return DISPATCH(e, vjp_tangent(e->stream, e->geo, e->cur, v.gobjP, v.segMN[n1 - 1 - n0], v.gP[h]));
             }));
         }
-        const RawPlanes& Utop = (s == v.nck - 1) ? e->cur : v.ckP[s + 1];
-        const RawPlanes& Gtop = (s == v.nck - 1) ? v.gobjP : v.gP[h];
-        const int o = (s == v.nck - 1) ? h : 1 - h;
-        TRY(run(e, LGT_K_VJP, "adjoint tangent", [&] {
-            return DISPATCH(e, vjp_tangent(e->stream, e->geo, Utop, Gtop, v.segMN[n1 - 1 - n0], v.gP[o]));
-        }));
-        h = o;
         for (int n = n1 - 1; n >= n0; n--) {
             const RawPlanes& Z = (n == n0) ? v.ckP[s] : v.segUP[n - n0 - 1];
-            const RawNorms* mprev = (n > n0) ? &v.segMN[n - 1 - n0] : nullptr;
+            // past the bottom of a segment (s >= 1) the |M| kept by the forward
+            const RawNorms* mprev = (n > n0) ? &v.segMN[n - 1 - n0] : (s > 0 ? &v.ckMN[s] : nullptr);
             TRY(run(e, LGT_K_VJP, "adjoint step", [&] {
                 return DISPATCH(e, vjp_step(e->stream, kgeo(e, LGT_K_VJP, 0), Z, v.gP[h], mprev, v.gP[1 - h], sp));
             }));

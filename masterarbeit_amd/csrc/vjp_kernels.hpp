@@ -38,12 +38,6 @@ __global__ void __launch_bounds__(BLOCK) k_vjp_tangent(Planes<T> U, Planes<T> G,
     qstore<T>(H.p[mu][p], i, h);
 }
 
-template <typename T>
-__device__ __forceinline__ Q<T> mul3(const Q<T>& a, const Q<T>& b, const Q<T>& c)
-{
-    return qmul(qmul(a, b), c);
-}
-
 // One backward step.  Z = input field of the step, Hin = tangent cotangent of
 // its output (H above).  Writes G = dL/dZ for the spatial links, or, when
 // Mprev is given (|M| of the step before), directly the tangent cotangent of
@@ -70,30 +64,36 @@ __global__ void __launch_bounds__(BLOCK) k_vjp_step(Planes<T> Z, Planes<T> Hin, 
 #pragma unroll
     for (int a = 1; a < 4; a++) {
         Q<T> acc = qzero<T>();
+        // the 24 gathers of both b != a are issued before any product: the
+        // latency of one batch is exposed once instead of per term
+        const int b0 = (a == 1) ? 2 : 1, b1 = (a == 3) ? 2 : 3;
+        Q<T> L[2][12];
 #pragma unroll
-        for (int b = 1; b < 4; b++) {
-            if (b == a) continue;
+        for (int j = 0; j < 2; j++) {
+            const int b = j ? b1 : b0;
+            Site ymba = shift(g, ym[b], a, 1);
+            L[j][0] = link(Z, g, b, y);       L[j][1] = link(Hin, g, b, y);
+            L[j][2] = link(Z, g, a, yp[b]);   L[j][3] = link(Hin, g, a, yp[b]);
+            L[j][4] = link(Z, g, b, yp[a]);   L[j][5] = link(Hin, g, b, yp[a]);
+            L[j][6] = link(Z, g, b, ym[b]);   L[j][7] = link(Hin, g, b, ym[b]);
+            L[j][8] = link(Z, g, a, ym[b]);   L[j][9] = link(Hin, g, a, ym[b]);
+            L[j][10] = link(Z, g, b, ymba);   L[j][11] = link(Hin, g, b, ymba);
+        }
+        asm volatile("" ::: "memory");
+#pragma unroll
+        for (int j = 0; j < 2; j++) {
             // forward staple: t1 + t5 = (H_b(y) Z_a(y+b) + Z_b(y) H_a(y+b)) Z_b(y+a)^+,
             //                 t4 = Z_b(y) Z_a(y+b) H_b(y+a)^+   (5 products, not 6)
-            {
-                Q<T> Zb_y = link(Z, g, b, y), Hb_y = link(Hin, g, b, y);
-                Q<T> Za_yb = link(Z, g, a, yp[b]), Ha_yb = link(Hin, g, a, yp[b]);
-                Q<T> up = qadd(qmul(Hb_y, Za_yb), qmul(Zb_y, Ha_yb));
-                Q<T> zz = qmul(Zb_y, Za_yb);
-                acc = qadd(acc, qmul_dag(up, link(Z, g, b, yp[a])));
-                acc = qadd(acc, qmul_dag(zz, link(Hin, g, b, yp[a])));
-            }
+            Q<T> up = qadd(qmul(L[j][1], L[j][2]), qmul(L[j][0], L[j][3]));
+            Q<T> zz = qmul(L[j][0], L[j][2]);
+            acc = qadd(acc, qmul_dag(up, L[j][4]));
+            acc = qadd(acc, qmul_dag(zz, L[j][5]));
             // backward staple: t2 + t3 = (Z_b(y-b)^+ H_a(y-b) + H_b(y-b)^+ Z_a(y-b)) Z_b(y-b+a),
             //                  t6 = Z_b(y-b)^+ Z_a(y-b) H_b(y-b+a)
-            {
-                Site ymba = shift(g, ym[b], a, 1);
-                Q<T> Zb_ymb = link(Z, g, b, ym[b]), Hb_ymb = link(Hin, g, b, ym[b]);
-                Q<T> Za_ymb = link(Z, g, a, ym[b]), Ha_ymb = link(Hin, g, a, ym[b]);
-                Q<T> lo = qadd(qmul(qdag(Zb_ymb), Ha_ymb), qmul(qdag(Hb_ymb), Za_ymb));
-                Q<T> zz = qmul(qdag(Zb_ymb), Za_ymb);
-                acc = qadd(acc, qmul(lo, link(Z, g, b, ymba)));
-                acc = qadd(acc, qmul(zz, link(Hin, g, b, ymba)));
-            }
+            Q<T> lo = qadd(qmul(qdag(L[j][6]), L[j][9]), qmul(qdag(L[j][7]), L[j][8]));
+            Q<T> zl = qmul(qdag(L[j][6]), L[j][8]);
+            acc = qadd(acc, qmul(lo, L[j][10]));
+            acc = qadd(acc, qmul(zl, L[j][11]));
         }
         Q<T> G = qadd(link(Hin, g, a, y), qscale(acc, sp));
         if (has_prev) {

@@ -1,0 +1,244 @@
+#!/usr/bin/env python3
+"""Physics studies on the GPU engine: thermalisation, FP32-storage precision,
+topology and a 96^4 production chain.
+
+    python -m masterarbeit_amd.studies c1|c2|c3|c5|all [--quick]
+
+Each study prints one JSON line on stdout (progress on stderr).  The field is
+resident on the device throughout; host transfers happen only where a
+configuration is handed from one engine to another (same configuration in FP32
+and FP64 storage) or restored after a destructive measurement.
+
+  c1  12^4, beta 3.09, cold start: 100 x (heatbath + overrelaxation) sweeps
+      with the plaquette after every sweep; 20 spatial APE steps (alpha 0.5,
+      sp 0.25); W(R,T) for R,T in 1..6.
+  c2  24^4, hot start (seed 1): 1000 sweeps in FP32 storage and in FP64;
+      <P> of both chains with autocorrelation-corrected errors; on a common
+      configuration every 100 sweeps, spatial APE to N = 100..500 and W for
+      R 1..10 x T 1..16 in both storages: the FP32 - FP64 differences.
+  c3  48^4, seed 7: 500 sweeps (FP64); 4D APE (sp 1/6) for 15 and 30 steps and
+      the improved charge Q, on the same configuration in FP64 and FP32
+      storage; |dQ|.
+  c5  96^4, FP32 storage, hot start (seed 3): 200 sweeps; 400 spatial APE
+      steps and W at R = T in {5, 7, 9}; 30 steps of 4D APE and Q; device
+      memory against the 288 GB of one MI355X.
+"""
+import argparse
+import json
+import sys
+import time
+
+import numpy as np
+
+from . import engine as E
+
+BETA = 3.09
+SP_SPATIAL = 0.25          # alpha = 0.5 in the reference's input convention
+SP_4D = 1.0 / 6.0
+
+
+def log(msg):
+    print(msg, file=sys.stderr, flush=True)
+
+
+def tau_int(x):
+    """Integrated autocorrelation time and error of the mean, summing the
+    normalised autocorrelation function until its first negative value
+    (LGT_Code/Autocorrelation/t_int.c:31-69): tau = 1/2 + sum_k Gamma_k/Gamma_0,
+    error = sqrt(2 tau) * sqrt(Gamma_0 / (N - 1))."""
+    x = np.asarray(x, dtype=np.float64)
+    n = len(x)
+
+    def gamma(k):
+        a, b = x[:n - k], x[k:]
+        return float(np.mean((a - a.mean()) * (b - b.mean())))
+
+    g0 = gamma(0)
+    if g0 <= 0.0:
+        return 0.5, 0.0
+    t = 0.5 * g0
+    for k in range(1, n):
+        gk = gamma(k)
+        if gk < 0.0:
+            break
+        t += gk
+    tau = t / g0
+    return tau, float(np.sqrt(2.0 * tau) * np.sqrt(g0 / (n - 1)))
+
+
+def loops(Rs, Ts):
+    R, T = np.meshgrid(np.asarray(Rs), np.asarray(Ts), indexing="ij")
+    return R.ravel(), T.ravel()
+
+
+def c1(quick=False):
+    L = (12, 12, 12, 12)
+    t0 = time.perf_counter()
+    with E.Engine(L, E.FP64, 1) as e:
+        e.init_field(0)
+        plaq = []
+        for _ in range(100):
+            e.update(1, 1, BETA)
+            plaq.append(e.plaquette())
+        R, T = loops(range(1, 7), range(1, 7))
+        e.ape(E.APE_SPATIAL, 20, SP_SPATIAL)
+        w = e.wilson_loops(R, T)
+    tail = np.array(plaq[50:])
+    tau, err = tau_int(tail)
+    return {"study": "c1", "lattice": L, "beta": BETA, "start": "cold", "precision": "fp64",
+            "plaquette_per_sweep": plaq, "plaquette_mean_50_100": float(tail.mean()), "err": err, "tau_int": tau,
+            "ape": {"steps": 20, "sp": SP_SPATIAL},
+            "wloops": {f"{r},{t}": float(v) for r, t, v in zip(R, T, w)}, "seconds": time.perf_counter() - t0}
+
+
+def c2(quick=False):
+    L = (24, 24, 24, 24)
+    nsweeps = 200 if quick else 1000
+    therm, every = 100, 100
+    R, T = loops(range(1, 11), range(1, 17))
+    Ns = [100, 200, 300, 400, 500]
+    t0 = time.perf_counter()
+    e64 = E.Engine(L, E.FP64, 1)
+    e32 = E.Engine(L, E.FP32, 1)
+    m64 = E.Engine(L, E.FP64, 2)      # measurement engines (APE is destructive)
+    m32 = E.Engine(L, E.FP32, 2)
+    e64.init_field(1)
+    e32.init_field(1)
+    p64, p32, dW = [], [], []
+    W64s = []
+    for s in range(1, nsweeps + 1):
+        e64.update(1, 1, BETA)
+        e32.update(1, 1, BETA)
+        p64.append(e64.plaquette())
+        p32.append(e32.plaquette())
+        if s > therm and s % every == 0:
+            U = e64.download()
+            m64.upload(U)
+            m32.upload(U)
+            d_n, w_n = [], []
+            done = 0
+            for n in Ns:
+                m64.ape(E.APE_SPATIAL, n - done, SP_SPATIAL)
+                m32.ape(E.APE_SPATIAL, n - done, SP_SPATIAL)
+                done = n
+                w64 = m64.wilson_loops(R, T)
+                w32 = m32.wilson_loops(R, T)
+                d_n.append(w32 - w64)
+                w_n.append(w64)
+            dW.append(np.array(d_n))
+            W64s.append(np.array(w_n))
+            log(f"[c2] sweep {s}: P64 {p64[-1]:.6f} P32 {p32[-1]:.6f} max|dW| {np.abs(dW[-1]).max():.3g}")
+    for x in (e64, e32, m64, m32):
+        x.close()
+    a64, a32 = np.array(p64[therm:]), np.array(p32[therm:])
+    t64, s64 = tau_int(a64)
+    t32, s32 = tau_int(a32)
+    dW = np.array(dW)        # [conf, N, loop]
+    W64s = np.array(W64s)
+    rel = np.abs(dW) / np.maximum(np.abs(W64s), 1e-300)
+    per_n = {}
+    for j, n in enumerate(Ns):
+        per_n[str(n)] = {
+            "max_abs_dW": float(np.abs(dW[:, j]).max()),
+            "max_rel_dW": float(rel[:, j].max()),
+            "rel_dW_W55": float(rel[:, j, list(zip(R, T)).index((5, 5))].max()),
+            "W55_mean_fp64": float(W64s[:, j, list(zip(R, T)).index((5, 5))].mean()),
+        }
+    dP = float(a32.mean() - a64.mean())
+    sig = float(np.hypot(s32, s64))
+    return {"study": "c2", "lattice": L, "beta": BETA, "sweeps": nsweeps, "thermalisation": therm,
+            "plaquette": {"fp64": float(a64.mean()), "err64": s64, "tau64": t64,
+                          "fp32": float(a32.mean()), "err32": s32, "tau32": t32,
+                          "dP": dP, "dP_over_sigma": dP / sig if sig > 0 else None},
+            "ape_wloops_same_config": {"configs": int(dW.shape[0]), "loops": "R 1..10 x T 1..16", "per_N": per_n},
+            "seconds": time.perf_counter() - t0}
+
+
+def c3(quick=False):
+    L = (48, 48, 48, 48)
+    nsweeps = 50 if quick else 500
+    t0 = time.perf_counter()
+    with E.Engine(L, E.FP64, 7) as e:
+        e.init_field(1)
+        for s in range(0, nsweeps, 50):
+            e.update(min(50, nsweeps - s), 1, BETA)
+            log(f"[c3] sweep {s + 50}: P {e.plaquette():.6f}")
+        plaq = e.plaquette()
+        U = e.download()
+    out = {"study": "c3", "lattice": L, "beta": BETA, "sweeps": nsweeps, "plaquette": plaq, "steps": {}}
+    res = {}
+    for prec, name in ((E.FP64, "fp64"), (E.FP32, "fp32")):
+        with E.Engine(L, prec, 7) as e:
+            e.upload(U)
+            done = 0
+            for n in (15, 30):
+                e.ape(E.APE_ALL, n - done, SP_4D)
+                done = n
+                q, qa = e.qtop(E.Q_IMPROVED)
+                res[(name, n)] = (q, qa)
+    del U
+    for n in (15, 30):
+        q64, qa64 = res[("fp64", n)]
+        q32, _ = res[("fp32", n)]
+        out["steps"][str(n)] = {"Q_fp64": q64, "Q_fp32": q32, "abs_dQ": abs(q32 - q64), "sum_abs_q_fp64": qa64}
+    out["seconds"] = time.perf_counter() - t0
+    return out
+
+
+def c5(quick=False):
+    L = (96, 96, 96, 96)
+    nsweeps = 20 if quick else 200
+    t0 = time.perf_counter()
+    with E.Engine(L, E.FP32, 3) as e:
+        e.init_field(1)
+        e.sync()
+        t1 = time.perf_counter()
+        for s in range(0, nsweeps, 20):
+            e.update(min(20, nsweeps - s), 1, BETA)
+            log(f"[c5] sweep {s + 20}: P {e.plaquette():.6f}")
+        e.sync()
+        t_sweeps = time.perf_counter() - t1
+        plaq = e.plaquette()
+        U = e.download()
+        t1 = time.perf_counter()
+        e.ape(E.APE_SPATIAL, 400, SP_SPATIAL)
+        e.sync()
+        t_ape = time.perf_counter() - t1
+        w = e.wilson_loops([5, 7, 9], [5, 7, 9])
+        log(f"[c5] W(5,5) {w[0]:.6g} W(7,7) {w[1]:.6g} W(9,9) {w[2]:.6g}")
+        e.upload(U)
+        del U
+        t1 = time.perf_counter()
+        e.ape(E.APE_ALL, 30, SP_4D)
+        e.sync()
+        t_ape4 = time.perf_counter() - t1
+        t1 = time.perf_counter()
+        q, qa = e.qtop(E.Q_IMPROVED)
+        t_q = time.perf_counter() - t1
+        dev = e.device_bytes()
+    vol = int(np.prod(L))
+    return {"study": "c5", "lattice": L, "beta": BETA, "precision": "fp32-store", "sweeps": nsweeps,
+            "plaquette": plaq, "sweep_s": t_sweeps / nsweeps,
+            "link_updates_per_s": 4.0 * vol * nsweeps / t_sweeps,
+            "ape400_s": t_ape, "wloops": {"5,5": w[0], "7,7": w[1], "9,9": w[2]},
+            "ape4d_30_s": t_ape4, "Q_improved": q, "sum_abs_q": qa, "qtop_s": t_q,
+            "device_bytes": dev, "device_fraction_of_288GB": dev / 288e9,
+            "seconds": time.perf_counter() - t0}
+
+
+def main():
+    ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
+    ap.add_argument("study", choices=["c1", "c2", "c3", "c5", "all"])
+    ap.add_argument("--quick", action="store_true", help="fewer sweeps (a functional check)")
+    a = ap.parse_args()
+    E.load_library()
+    todo = ["c1", "c2", "c3", "c5"] if a.study == "all" else [a.study]
+    for s in todo:
+        log(f"[{s}] start")
+        r = globals()[s](a.quick)
+        print(json.dumps(r), flush=True)
+        log(f"[{s}] done in {r['seconds']:.1f} s")
+
+
+if __name__ == "__main__":
+    main()

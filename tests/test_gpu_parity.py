@@ -293,11 +293,13 @@ def test_update_matches_oracle_chain(lgt, L, n_or):
 def test_results_independent_of_tile_order(lgt, prec):
     """Every tile order (XCD-contiguous, time-fastest, both, neither) gives
     bit-identical sweeps, smearing and observables; repeated runs too."""
-    L = (8, 16, 16, 16)    # a time slice holds 8 whole tiles: every order applies (groups of 8)
+    L = (8, 16, 16, 16)    # a time slice holds 8 whole tiles: every order applies
     res = []
-    for order in (-1, 0, 1, 2, 3, 4, 5, 7, -1):
+    for order, group in ((-1, 0), (0, 0), (1, 0), (2, 0), (3, 0), (4, 0), (5, 0), (7, 0), (5, 2), (5, 4), (5, 8),
+                         (-1, 0)):
         with engine(lgt, L, prec, seed=7) as e:
             e.set_option(0, order)
+            e.set_option(lgt.engine.OPT_GROUP, group)
             e.init_field(1)
             e.update(3, 1, 3.09)
             e.ape(0, 2, 0.25)
@@ -305,6 +307,13 @@ def test_results_independent_of_tile_order(lgt, prec):
     for r in res[1:]:
         np.testing.assert_array_equal(r[0], res[0][0])
         assert r[1:] == res[0][1:]
+
+
+def test_group_size_must_divide_time_slice(lgt):
+    with engine(lgt, (8, 16, 16, 16), 0, seed=7) as e:
+        with pytest.raises(lgt.LgtError):
+            e.set_option(lgt.engine.OPT_GROUP, 3)
+        e.set_option(lgt.engine.OPT_GROUP, 4)
 
 
 # ---------------------------------------------------------------------------
