@@ -12,8 +12,9 @@ recorded around every launch of a second run of the same K steps (the event
 records would otherwise add gaps to the timed steps).
 
 Also reported (extra keys): spatial APE steps/s, Wilson-loop and charge times,
-and the reference CPU code timed on this host (one core) on one sweep of the
-same 48^4 configuration.
+the APE-400 adjoint/forward time ratio, the FP64 field's host round trip, and
+the reference CPU code timed on this host on one sweep of the same 48^4
+configuration (one core; then --cpu-procs independent processes at once).
 
 Usage: python bench.py [--gpus 1] [--steps K] [--warmup W]
 """
@@ -162,25 +163,40 @@ def time_observables(e):
     return out
 
 
-def cpu_baseline(e64, L, beta):
+# one reference sweep in a fresh interpreter (no GPU): prints its own duration
+_CPU_WORKER = """
+import sys, time, numpy as np
+sys.path.insert(0, {tests!r})
+from oracle_lib import RefLib
+ref = RefLib({name!r}); ref.setup({beta!r}, 1)
+ref.set_field(np.load({path!r}, mmap_mode="r"))
+t0 = time.perf_counter(); ref.lib.ref_update(1); print(time.perf_counter() - t0, flush=True)
+"""
+
+
+def cpu_baseline(e64, L, beta, procs):
     """The reference's own heatbath sweep (LGT_Code/modules/update/update.c:102-176,
-    compiled from its sources into oracle/_ref) on one core, on the thermalised
-    FP64 configuration of the GPU run."""
-    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    compiled from its sources into oracle/_ref) on the thermalised FP64
+    configuration of the GPU run: once on one core (the reference is
+    single-threaded), then as `procs` independent processes at once, the
+    aggregate over the host's cores."""
+    import subprocess
+    import tempfile
+    tests = os.path.join(ROOT, "tests")
+    sys.path.insert(0, tests)
     from oracle_lib import REF_SIZES, RefLib
     name = {v: k for k, v in REF_SIZES.items()}.get(tuple(L))
     if name is None or not RefLib.available(name):
         return None
+    vol = int(np.prod(L))
     ref = RefLib(name)
     ref.setup(beta, 1)
     U = e64.download()
     ref.set_field(U)
-    del U
     t0 = time.perf_counter()
     ref.lib.ref_update(1)
     dt = time.perf_counter() - t0
-    vol = int(np.prod(L))
-    return {
+    out = {
         "value": 4.0 * vol / dt,
         "unit": "link updates/s",
         "cores": 1,
@@ -188,6 +204,41 @@ def cpu_baseline(e64, L, beta):
         "sample": f"1 lexicographic heatbath sweep (gaugefieldUpdate, no overrelaxation) of the thermalised "
                   f"{'x'.join(map(str, L))} FP64 configuration, {dt:.1f} s on 1 host core",
     }
+    if procs > 1:
+        with tempfile.TemporaryDirectory() as tmp:
+            path = os.path.join(tmp, "field.npy")
+            np.save(path, U)
+            del U
+            code = _CPU_WORKER.format(tests=tests, name=name, beta=beta, path=path)
+            t0 = time.perf_counter()
+            ps = [subprocess.Popen([sys.executable, "-c", code], stdout=subprocess.PIPE, text=True)
+                  for _ in range(procs)]
+            times = [float(p.communicate()[0].split()[-1]) for p in ps]
+            wall = time.perf_counter() - t0
+        out["all_cores"] = {
+            "value": 4.0 * vol * procs / wall,
+            "unit": "link updates/s",
+            "processes": procs,
+            "host_cpus": os.cpu_count(),
+            "sweep_s_per_process": [round(t, 2) for t in times],
+            "wall_s": wall,
+        }
+    return out
+
+
+def time_host_transfer(e64):
+    """Host <-> device round trip of the FP64 field in the reference's layout
+    (what the drop-in entry points pay per call when the field lives in the
+    program's host array)."""
+    U = e64.download()
+    t0 = time.perf_counter()
+    e64.upload(U)
+    e64.sync()
+    t_up = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    U = e64.download()
+    t_down = time.perf_counter() - t0
+    return {"bytes": int(U.nbytes), "upload_s": t_up, "download_s": t_down}
 
 
 def load_traffic(kernel, workload):
@@ -210,6 +261,8 @@ def main():
     ap.add_argument("--seed", type=int, default=7)
     ap.add_argument("--nplaq", type=int, default=50)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--cpu-procs", type=int, default=16,
+                    help="independent reference processes for the all-cores CPU figure (1: skip)")
     ap.add_argument("--vjp-steps", type=int, default=400)
     args = ap.parse_args()
     if args.gpus != 1:
@@ -254,9 +307,10 @@ def main():
     diff = float(p32.mean() - p64.mean()) if len(p32) else None
 
     base = None
+    xfer = time_host_transfer(e64)
     if not args.no_cpu_baseline:
-        log("[bench] timing the reference CPU sweep (one core) ...")
-        base = cpu_baseline(e64, L, args.beta)
+        log("[bench] timing the reference CPU sweep (one core, then all cores) ...")
+        base = cpu_baseline(e64, L, args.beta, args.cpu_procs)
     e64.close()
 
     out = {
@@ -314,6 +368,7 @@ def main():
         "stream_pass": stream32,
         "observables_s": {"fp32": obs32, "fp64": obs64},
         "device_bytes": {"fp32": r32["device_bytes"], "fp64": r64["device_bytes"]},
+        "host_transfer_fp64": xfer,
     }
     print(json.dumps(out), flush=True)
 

@@ -424,7 +424,16 @@ __global__ void __launch_bounds__(BLOCK) k_overrelax(Planes<T> P, Geo g, int p)
     if (i >= g.vh) return;
     Site s = cb_site(g, i, p);
     Q<T> A = update_staple<T, MU>(P, g, s, p);
-    Q<T> U = qload<T>(P.p[MU][p], i);
+    // the link's own old value is read once by this launch and by no staple:
+    // in FP32 the nontemporal hint keeps it from displacing gathered
+    // neighbours in L2 (-4 % at 48^4; in FP64 it measured slower)
+    Q<T> U;
+    if constexpr (sizeof(T) == 4) {
+        typename Store<T>::V v = __builtin_nontemporal_load(&P.p[MU][p][i]);
+        U = Q<T>{v.x, v.y, v.z, v.w};
+    } else {
+        U = qload<T>(P.p[MU][p], i);
+    }
     qstore<T>(P.p[MU][p], i, overrelax_link(A, U));
 }
 
