@@ -14,14 +14,25 @@
  * declarations below repeat the reference's struct layouts member for member
  * (DIM = 4, SUN = 2); only their layout matters, not these names' origin.
  *
- * Coherence: the host field (pu) is the authority between calls.  Every entry
- * point uploads the field it reads and downloads the field it changes, so a
- * caller that reads pu directly (writeConfig) always sees current links.
+ * Coherence of pu with the device field:
+ *  - eager (default): every entry point uploads the field it reads and
+ *    downloads the field it changes, so pu is current after every call;
+ *  - lazy: the device field is authoritative; entry points upload pu only when
+ *    the program changed it and download nothing.  The program is linked with
+ *    -Wl,--wrap=initGaugeField,--wrap=readConfig,--wrap=readConfigForThermalisation,
+ *    --wrap=writeConfig,--wrap=copyGaugeField,--wrap=releaseGaugeField so
+ *    that its own field-lifecycle functions (modules.h:119-142) pass through
+ *    the __wrap_ functions below, which download before pu is read and mark
+ *    pu changed after it is written.  The first intercepted call selects lazy
+ *    mode.  A program that writes pu by other means calls
+ *    lgt_compat_host_modified(); one that reads pu by other means calls
+ *    lgt_compat_sync_host() first.
  *
  * Environment: LGT_PRECISION=fp64|fp32 (default fp64), LGT_N_OR=<n> (number
  * of overrelaxation sweeps after each heatbath sweep, default 0 as in the
- * reference).  The engine's RNG seed is drawn from the program's own ranlxd
- * stream when the engine is created (first call).
+ * reference), LGT_COMPAT_SYNC=eager|lazy (overrides the mode).  The engine's
+ * RNG seed is drawn from the program's own ranlxd stream when the engine is
+ * created (first call).
  */
 #ifndef LGT_COMPAT_H
 #define LGT_COMPAT_H
@@ -89,6 +100,26 @@ void project_gfield_to_sun(su2mat *u[][4]);               /* modules.h:176, sun_
 
 /* no reference counterpart: releases the engine (also registered with atexit) */
 void lgt_compat_release(void);
+/* no reference counterpart: pu was written by the program outside the
+ * intercepted functions (lazy mode uploads it at the next entry point) */
+void lgt_compat_host_modified(void);
+/* no reference counterpart: make pu current (lazy mode downloads if needed) */
+void lgt_compat_sync_host(void);
+/* no reference counterpart: field transfers so far (host to device, back) */
+void lgt_compat_transfers(long long *uploads, long long *downloads);
+
+/* no reference counterpart: select lazy coherence (called by the
+ * interceptors of liblgt_compat_wrap.a; LGT_COMPAT_SYNC=eager overrides) */
+void lgt_compat_lazy(void);
+
+/* interceptors of the program's field-lifecycle functions for lazy mode, in
+ * liblgt_compat_wrap.a (link the program with it and -Wl,--wrap=<name>) */
+void __wrap_initGaugeField(int start);
+void __wrap_releaseGaugeField(void);
+void __wrap_readConfig(char *name);
+void __wrap_readConfigForThermalisation(char *name);
+void __wrap_writeConfig(char *name);
+void __wrap_copyGaugeField(su2mat *u1[][4], su2mat *u2[][4]);
 
 #ifdef __cplusplus
 }

@@ -1,8 +1,21 @@
 /*
  * lgt_compat.c -- the reference's entry points (include/lgt_compat.h) on top of
- * the GPU engine (include/lgt_su2.h).  C host code; every call gathers the
- * field through the program's pointer table, runs on the device and scatters
- * changed links back, so pu stays the authority between calls.
+ * the GPU engine (include/lgt_su2.h).  C host code.
+ *
+ * Coherence of the program's host field pu with the device field:
+ *   eager (default): every call uploads the field it reads and downloads the
+ *     field it changes, so pu is always current;
+ *   lazy: the device field is authoritative between calls.  An entry point
+ *     uploads pu only when the host copy changed since the last transfer and
+ *     downloads nothing; the field-lifecycle functions of the program
+ *     (initGaugeField, readConfig, readConfigForThermalisation, writeConfig,
+ *     copyGaugeField, releaseGaugeField) are intercepted with the linker's
+ *     --wrap (lgt_compat_wrap.c, a static archive linked into the program)
+ *     and download before the program reads pu (writeConfig, a copy out of
+ *     pu) or mark the host copy changed after it writes pu.  Lazy mode
+ *     starts with the first intercepted call (a program linked with --wrap
+ *     always calls initGaugeField first) or with LGT_COMPAT_SYNC=lazy;
+ *     LGT_COMPAT_SYNC=eager keeps eager mode.
  */
 #include "../../include/lgt_compat.h"
 
@@ -19,6 +32,12 @@ static int64_t g_vol = 0;
 static double *g_buf = NULL;
 static int g_nor = 0;
 
+/* coherence state */
+static int g_mode = -1;          /* -1: not chosen yet, 0: eager, 1: lazy */
+static int g_host_dirty = 1;     /* pu changed since the last transfer */
+static int g_dev_dirty = 0;      /* device field newer than pu */
+static long long g_uploads = 0, g_downloads = 0;
+
 static char NAME[] = "lgt_compat [lgt_compat.c]";
 
 static void check(int rc, const char *what)
@@ -29,12 +48,31 @@ static void check(int rc, const char *what)
     }
 }
 
+static int lazy(void)
+{
+    if (g_mode < 0) {
+        const char *m = getenv("LGT_COMPAT_SYNC");
+        g_mode = (m && !strcmp(m, "lazy")) ? 1 : 0;
+    }
+    return g_mode == 1;
+}
+
+void lgt_compat_lazy(void)
+{
+    if (g_mode < 0) {
+        const char *m = getenv("LGT_COMPAT_SYNC");
+        g_mode = (m && !strcmp(m, "eager")) ? 0 : 1;
+    }
+}
+
 void lgt_compat_release(void)
 {
     if (g_eng) lgt_destroy(g_eng);
     g_eng = NULL;
     free(g_buf);
     g_buf = NULL;
+    g_host_dirty = 1;
+    g_dev_dirty = 0;
 }
 
 static void engine(void)
@@ -65,6 +103,8 @@ static void engine(void)
         char fmt[] = "cannot allocate the %lld-site transfer buffer";
         error(1, NAME, fmt, (long long)g_vol);
     }
+    g_host_dirty = 1;
+    g_dev_dirty = 0;
     atexit(lgt_compat_release);
 }
 
@@ -74,6 +114,7 @@ static void upload(su2mat *u[][4])
     for (int64_t s = 0; s < g_vol; s++)
         for (int d = 0; d < 4; d++) memcpy(g_buf + 16 * s + 4 * d, u[s][d], 4 * sizeof(double));
     check(lgt_upload(g_eng, g_buf), "lgt_upload");
+    g_uploads++;
 }
 
 static void download(su2mat *u[][4])
@@ -81,8 +122,63 @@ static void download(su2mat *u[][4])
     check(lgt_download(g_eng, g_buf), "lgt_download");
     for (int64_t s = 0; s < g_vol; s++)
         for (int d = 0; d < 4; d++) memcpy(u[s][d], g_buf + 16 * s + 4 * d, 4 * sizeof(double));
+    g_downloads++;
 }
 
+/* the device holds pu's current links */
+static void pu_on_device(void)
+{
+    engine();
+    if (!lazy() || g_host_dirty) {
+        upload(pu);
+        g_host_dirty = 0;
+    }
+}
+
+/* pu holds the device's current links */
+static void pu_on_host(void)
+{
+    if (g_eng && g_dev_dirty) {
+        download(pu);
+        g_dev_dirty = 0;
+    }
+}
+
+/* the device field (pu's links) was changed on the device */
+static void device_changed(void)
+{
+    if (lazy()) {
+        g_dev_dirty = 1;
+    } else {
+        download(pu);
+    }
+}
+
+/* a field other than pu goes through the device: pu's device copy is lost */
+static void run_on_other_field(su2mat *u[][4], int (*op)(int, double), int a, double b)
+{
+    pu_on_host();
+    upload(u);
+    check(op(a, b), "lgt call");
+    download(u);
+    g_host_dirty = 1;
+}
+
+void lgt_compat_host_modified(void)
+{
+    g_host_dirty = 1;
+    g_dev_dirty = 0;
+}
+
+void lgt_compat_sync_host(void) { pu_on_host(); }
+
+void lgt_compat_transfers(long long *uploads, long long *downloads)
+{
+    if (uploads) *uploads = g_uploads;
+    if (downloads) *downloads = g_downloads;
+}
+
+/* ---- the reference's entry points ---- */
 void gaugefieldUpdate(int iup, int nup, int utype, int stype)
 {
     char m1[] = "Only the heatbath (SIM_TYPE=1) runs on the GPU engine!";
@@ -90,10 +186,10 @@ void gaugefieldUpdate(int iup, int nup, int utype, int stype)
     error(utype != 1, NAME, m1);
     error(stype != 0 && stype != 1, NAME, m2);
     double t1 = getTime();
-    upload(pu);
+    pu_on_device();
     double acc = 0.0;
     check(lgt_update(g_eng, nup, g_nor, runParams.beta, &acc), "lgt_update");
-    download(pu);
+    device_changed();
     double t2 = getTime();
     char fmt[] = "\nUpdate %d done:\n(algorithm %d; sweep %d; acc=%.3f)\nupdate took %.9f seconds\n";
     logging(fmt, iup, utype, stype, acc, t2 - t1);
@@ -102,7 +198,7 @@ void gaugefieldUpdate(int iup, int nup, int utype, int stype)
 double plaquette(void)
 {
     double p = 0.0;
-    upload(pu);
+    pu_on_device();
     check(lgt_plaquette(g_eng, &p), "lgt_plaquette");
     return p;
 }
@@ -113,11 +209,21 @@ double gaugeAction(void)
     return runParams.beta * 6 * (double)g_vol * (1. - p);
 }
 
+static int ape_op(int mode_ns, double sp)
+{
+    return lgt_ape(g_eng, mode_ns & 3, mode_ns >> 2, sp);
+}
+
 static void ape(int mode, int ns, double sp, su2mat *u[][4])
 {
-    upload(u);
-    check(lgt_ape(g_eng, mode, ns, sp), "lgt_ape");
-    download(u);
+    if (u == pu) {
+        pu_on_device();
+        check(lgt_ape(g_eng, mode, ns, sp), "lgt_ape");
+        device_changed();
+    } else {
+        engine();
+        run_on_other_field(u, ape_op, mode | (ns << 2), sp);
+    }
 }
 
 void smearing_APE_all(int ns, double sp, su2mat *u[][4]) { ape(LGT_APE_ALL, ns, sp, u); }
@@ -140,7 +246,7 @@ void measureWilsonLoop(double *w)
     char m[] = "Lattice too small to measure loops!";
     char nm[] = "meas_wilson";
     error(latParams.linearExtent[0] - 1 < measParams.tf, nm, m);
-    upload(pu);
+    pu_on_device();
     check(lgt_wilson_loops(g_eng, nloop, measParams.rExtents, measParams.tExtents, w), "lgt_wilson_loops");
 }
 
@@ -150,7 +256,7 @@ double topologicalCharge(void)
     char m2[] = "measurement done (took %.3e sec)\n";
     double t1 = getTime(), q = 0.0;
     logging(m1);
-    upload(pu);
+    pu_on_device();
     check(lgt_qtop(g_eng, LGT_Q_IMPROVED, &q, NULL, NULL), "lgt_qtop");
     logging(m2, getTime() - t1);
     return q;
@@ -158,13 +264,25 @@ double topologicalCharge(void)
 
 void meas_topologicalcharge(double *q)
 {
-    upload(pu);
+    pu_on_device();
     check(lgt_qtop(g_eng, LGT_Q_CLOVER, q, NULL, NULL), "lgt_qtop");
+}
+
+static int project_op(int unused_a, double unused_b)
+{
+    (void)unused_a;
+    (void)unused_b;
+    return lgt_project(g_eng);
 }
 
 void project_gfield_to_sun(su2mat *u[][4])
 {
-    upload(u);
-    check(lgt_project(g_eng), "lgt_project");
-    download(u);
+    if (u == pu) {
+        pu_on_device();
+        check(lgt_project(g_eng), "lgt_project");
+        device_changed();
+    } else {
+        engine();
+        run_on_other_field(u, project_op, 0, 0.0);
+    }
 }

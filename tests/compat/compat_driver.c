@@ -59,13 +59,18 @@ void ranlxd(double r[], int n)
     for (int i = 0; i < n; i++) r[i] = 0.25 * (double)((k++ % 3) + 1);
 }
 
-static void write_field(const char *dir, const char *name, const double *buf, long vol)
+/* field lifecycle (compat_lifecycle.c, a separate object as in the reference) */
+long driver_vol = 0;
+void initGaugeField(int start);
+void releaseGaugeField(void);
+void readConfig(char *name);
+void writeConfig(char *name);
+
+static void write_field(const char *dir, const char *name)
 {
     char path[1024];
     snprintf(path, sizeof(path), "%s/%s", dir, name);
-    FILE *f = fopen(path, "wb");
-    fwrite(buf, sizeof(double), 16 * (size_t)vol, f);
-    fclose(f);
+    writeConfig(path);
 }
 
 int main(int argc, char **argv)
@@ -82,13 +87,9 @@ int main(int argc, char **argv)
     if (vol > MAXVOL) return 2;
     const char *out = argv[6];
     snprintf(LOG_FILE, sizeof(LOG_FILE), "%s/log.txt", out);
-    double *field = malloc(sizeof(double) * 16 * vol), *orig = malloc(sizeof(double) * 16 * vol);
-    FILE *f = fopen(argv[5], "rb");
-    if (!f || fread(orig, sizeof(double), 16 * vol, f) != (size_t)(16 * vol)) return 3;
-    fclose(f);
-    memcpy(field, orig, sizeof(double) * 16 * vol);
-    for (long n = 0; n < vol; n++)
-        for (int d = 0; d < 4; d++) pu[n][d] = (su2mat *)(field + 16 * n + 4 * d);
+    driver_vol = vol;
+    initGaugeField(0);
+    readConfig(argv[5]);
     runParams.beta = 3.09;
 
     char path[1024];
@@ -115,19 +116,23 @@ int main(int argc, char **argv)
     fprintf(s, "QCLOV %.17g\n", qc);
 
     smearing_APE_spatial(3, 0.25, pu);
-    write_field(out, "ape_spatial.bin", field, vol);
+    write_field(out, "ape_spatial.bin");
     smearing_APE_all(2, 1.0 / 6.0, pu);
-    write_field(out, "ape_all.bin", field, vol);
+    write_field(out, "ape_all.bin");
     smearing_APE_temporal(1, 0.3, pu);
-    write_field(out, "ape_temporal.bin", field, vol);
+    write_field(out, "ape_temporal.bin");
 
-    memcpy(field, orig, sizeof(double) * 16 * vol);
+    readConfig(argv[5]);
     gaugefieldUpdate(1, 2, 1, 0);
-    write_field(out, "update.bin", field, vol);
+    write_field(out, "update.bin");
     project_gfield_to_sun(pu);
-    write_field(out, "project.bin", field, vol);
+    write_field(out, "project.bin");
     fprintf(s, "plaq_after %.17g\n", plaquette());
+    long long up = 0, down = 0;
+    lgt_compat_transfers(&up, &down);
+    fprintf(s, "uploads %lld\ndownloads %lld\n", up, down);
     fclose(s);
+    releaseGaugeField();
     lgt_compat_release();
     return 0;
 }

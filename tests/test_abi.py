@@ -36,7 +36,8 @@ def test_compat_exports_reference_names():
         pytest.fail("liblgt_compat.so not built (run __graft_entry__.build())")
     text = open(os.path.join(ROOT, "include", "lgt_compat.h")).read()
     text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
-    exported = set(re.findall(r"^(?:void|double)\s+(\w+)\s*\(", text, flags=re.M))
+    declared_names = set(re.findall(r"^(?:void|double)\s+(\w+)\s*\(", text, flags=re.M))
+    exported = {n for n in declared_names if not n.startswith("__wrap_")}
     assert {"gaugefieldUpdate", "plaquette", "measureWilsonLoop", "topologicalCharge",
             "smearing_APE_spatial", "meas_topologicalcharge"} <= exported
     nm = subprocess.run(["nm", "-D", path], capture_output=True, text=True, check=True).stdout
@@ -46,6 +47,12 @@ def test_compat_exports_reference_names():
     undefined = {l.split()[-1].split("@")[0] for l in nm.splitlines() if " U " in l}
     assert exported <= defined
     assert {"pu", "runParams", "latParams", "measParams", "logging", "error", "getTime", "ranlxd"} <= undefined
+    # the lifecycle interceptors live in the static archive linked into the program
+    wraps = {n for n in declared_names if n.startswith("__wrap_")}
+    assert len(wraps) == 6
+    archive = os.path.join(ROOT, "masterarbeit_amd", "liblgt_compat_wrap.a")
+    nm = subprocess.run(["nm", archive], capture_output=True, text=True, check=True).stdout
+    assert wraps <= {l.split()[-1] for l in nm.splitlines() if " T " in l}
 
 
 def test_python_binding_covers_header():

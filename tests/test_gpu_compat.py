@@ -14,6 +14,7 @@ pytestmark = pytest.mark.gpu
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 DRIVER = os.path.join(ROOT, "tests", "compat", "_build", "compat_driver")
+DRIVER_LAZY = os.path.join(ROOT, "tests", "compat", "_build", "compat_driver_lazy")
 SEED = ((1 << 30) << 32) | (1 << 31)     # drawn from the driver's ranlxd: 0.25, 0.5
 
 
@@ -28,9 +29,13 @@ def read_summary(path):
     return out
 
 
-@pytest.mark.parametrize("prec", ["fp64", "fp32"])
-def test_reference_entry_points_through_compat(tmp_path, prec):
-    if not os.path.exists(DRIVER):
+@pytest.mark.parametrize("prec,sync", [("fp64", "eager"), ("fp32", "eager"), ("fp64", "lazy"), ("fp32", "lazy")])
+def test_reference_entry_points_through_compat(tmp_path, prec, sync):
+    """Both coherence modes give the same results; the lazy build (the
+    program's lifecycle functions wrapped at link time) moves the field only
+    when the program itself reads or writes it."""
+    driver = DRIVER if sync == "eager" else DRIVER_LAZY
+    if not os.path.exists(driver):
         pytest.fail("compat driver not built (run __graft_entry__.build())")
     L = (6, 4, 4, 8)
     o = Oracle(L)
@@ -38,7 +43,7 @@ def test_reference_entry_points_through_compat(tmp_path, prec):
     o.engine_sweep(U, 3.09, 1, 21, 0)
     U.tofile(tmp_path / "field.bin")
     env = dict(os.environ, LGT_N_OR="1", LGT_PRECISION=prec)
-    r = subprocess.run([DRIVER, *map(str, L), str(tmp_path / "field.bin"), str(tmp_path)], env=env,
+    r = subprocess.run([driver, *map(str, L), str(tmp_path / "field.bin"), str(tmp_path)], env=env,
                        capture_output=True, text=True, timeout=120)
     assert r.returncode == 0, r.stderr
     s = read_summary(tmp_path / "summary.txt")
@@ -68,6 +73,11 @@ def test_reference_entry_points_through_compat(tmp_path, prec):
     else:
         G = load("update.bin")
         np.testing.assert_allclose(np.linalg.norm(G, axis=-1), 1.0, atol=3e-7)
+    if sync == "lazy":
+        # two uploads (after each readConfig), a download per writeConfig
+        assert (s["uploads"], s["downloads"]) == (2, 5)
+    else:
+        assert (s["uploads"], s["downloads"]) == (11, 5)
     log = open(tmp_path / "log.txt").read()
     assert "Update 1 done:\n(algorithm 1; sweep 0; acc=" in log and "update took" in log
     assert "calculating topological charge" in log
