@@ -110,6 +110,20 @@ int lgt_init_field(lgt_engine *e, int start);
 int lgt_upload(lgt_engine *e, const double *aos);
 int lgt_download(lgt_engine *e, double *aos);
 
+/* writeConfig / readConfig (LGT_Code/modules/io/config_IO.c:40-117,537-621):
+ * the reference's binary configuration file, little-endian: int32 DIM = 4,
+ * int32 SUN = 2, int32 extents[4] (t, x, y, z), double plaquette, then per
+ * site (z fastest) and direction the link's c0..c3 as doubles.
+ * lgt_write_config stores the engine's field with its plaquette.
+ * lgt_read_config checks DIM, SUN and the extents (LGT_EINVAL otherwise),
+ * loads the field and recomputes the plaquette; it returns LGT_EINVAL ("plaquette
+ * test failed", the field stays loaded) when it differs from the file's by
+ * more than sqrt(6 VOL) eps, eps = DBL_EPSILON (FP64 engine, the reference's
+ * test) or FLT_EPSILON (FP32 storage).  plaq_file / plaq_calc (nullable)
+ * return the two values. */
+int lgt_write_config(lgt_engine *e, const char *path);
+int lgt_read_config(lgt_engine *e, const char *path, double *plaq_file, double *plaq_calc);
+
 /* ---- update -------------------------------------------------------------- */
 
 /* gaugefieldUpdate(iup, nup, utype=1, stype) (LGT_Code/modules/update/

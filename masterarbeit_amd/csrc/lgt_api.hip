@@ -3,10 +3,13 @@
 // they are instantiated per precision in lgt_f64.hip / lgt_f32.hip.
 #include <hip/hip_runtime.h>
 
+#include <cfloat>
 #include <cmath>
 #include <cstdarg>
 #include <cstdio>
+#include <cstdint>
 #include <cstring>
+#include <new>
 #include <map>
 #include <string>
 #include <vector>
@@ -511,6 +514,85 @@ int lgt_download(lgt_engine* e, double* aos)
     TRY(run(e, LGT_K_FIELD, "to_aos", [&] { return DISPATCH(e, to_aos(e->stream, e->geo, e->cur, (double*)e->staging)); }));
     HIP_TRY(hipMemcpyAsync(aos, e->staging, bytes, hipMemcpyDeviceToHost, e->stream), "hipMemcpyAsync D2H");
     HIP_TRY(hipStreamSynchronize(e->stream), "hipStreamSynchronize");
+    return LGT_OK;
+}
+
+// ---- configuration files (config_IO.c:40-117,537-621) ----
+static_assert(sizeof(int32_t) == 4 && sizeof(double) == 8, "config format");
+
+namespace {
+bool host_little_endian()
+{
+    const uint16_t one = 1;
+    return *(const uint8_t*)&one == 1;
+}
+
+struct FileCloser {
+    FILE* f;
+    ~FileCloser() { if (f) fclose(f); }
+};
+}  // namespace
+
+int lgt_write_config(lgt_engine* e, const char* path)
+{
+    if (!e || !path) return fail(LGT_EINVAL, "lgt_write_config: null argument");
+    if (!host_little_endian()) return fail(LGT_EINVAL, "lgt_write_config: big-endian host");
+    double plaq = 0.0;
+    TRY(lgt_plaquette(e, &plaq));
+    std::vector<double> aos;
+    try {
+        aos.resize(16 * (size_t)e->vol);
+    } catch (const std::bad_alloc&) {
+        return fail(LGT_ENOMEM, "lgt_write_config: host buffer");
+    }
+    TRY(lgt_download(e, aos.data()));
+    FileCloser fc{fopen(path, "wb")};
+    if (!fc.f) return fail(LGT_EINVAL, "lgt_write_config: cannot create %s", path);
+    const int32_t info[2] = {4, 2};
+    const int32_t ls[4] = {e->L[0], e->L[1], e->L[2], e->L[3]};
+    bool ok = fwrite(info, sizeof(int32_t), 2, fc.f) == 2 && fwrite(ls, sizeof(int32_t), 4, fc.f) == 4 &&
+              fwrite(&plaq, sizeof(double), 1, fc.f) == 1 && fwrite(aos.data(), sizeof(double), aos.size(), fc.f) == aos.size();
+    if (!ok) return fail(LGT_EINVAL, "lgt_write_config: write error on %s", path);
+    if (fclose(fc.f) != 0) {
+        fc.f = nullptr;
+        return fail(LGT_EINVAL, "lgt_write_config: write error on %s", path);
+    }
+    fc.f = nullptr;
+    return LGT_OK;
+}
+
+int lgt_read_config(lgt_engine* e, const char* path, double* plaq_file, double* plaq_calc)
+{
+    if (!e || !path) return fail(LGT_EINVAL, "lgt_read_config: null argument");
+    if (!host_little_endian()) return fail(LGT_EINVAL, "lgt_read_config: big-endian host");
+    FileCloser fc{fopen(path, "rb")};
+    if (!fc.f) return fail(LGT_EINVAL, "lgt_read_config: cannot open %s", path);
+    int32_t info[2], ls[4];
+    double plaq0 = 0.0;
+    if (fread(info, sizeof(int32_t), 2, fc.f) != 2 || fread(ls, sizeof(int32_t), 4, fc.f) != 4 ||
+        fread(&plaq0, sizeof(double), 1, fc.f) != 1)
+        return fail(LGT_EINVAL, "lgt_read_config: read error on %s", path);
+    if (info[0] != 4 || info[1] != 2) return fail(LGT_EINVAL, "lgt_read_config: DIM %d SUN %d, expected 4 2", info[0], info[1]);
+    for (int d = 0; d < 4; d++)
+        if (ls[d] != e->L[d])
+            return fail(LGT_EINVAL, "lgt_read_config: lattice %dx%dx%dx%d, engine %dx%dx%dx%d", ls[0], ls[1], ls[2], ls[3],
+                        e->L[0], e->L[1], e->L[2], e->L[3]);
+    std::vector<double> aos;
+    try {
+        aos.resize(16 * (size_t)e->vol);
+    } catch (const std::bad_alloc&) {
+        return fail(LGT_ENOMEM, "lgt_read_config: host buffer");
+    }
+    if (fread(aos.data(), sizeof(double), aos.size(), fc.f) != aos.size())
+        return fail(LGT_EINVAL, "lgt_read_config: %s is short", path);
+    TRY(lgt_upload(e, aos.data()));
+    double plaq = 0.0;
+    TRY(lgt_plaquette(e, &plaq));
+    if (plaq_file) *plaq_file = plaq0;
+    if (plaq_calc) *plaq_calc = plaq;
+    const double eps = std::sqrt(6.0 * (double)e->vol) * (e->prec == LGT_FP64 ? DBL_EPSILON : (double)FLT_EPSILON);
+    if (!(std::fabs(plaq - plaq0) <= eps))
+        return fail(LGT_EINVAL, "lgt_read_config: plaquette test failed (file %.16g, field %.16g)", plaq0, plaq);
     return LGT_OK;
 }
 

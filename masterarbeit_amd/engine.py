@@ -40,6 +40,8 @@ SIGNATURES = {
     "lgt_init_field": (C.c_int, [C.c_void_p, C.c_int]),
     "lgt_upload": (C.c_int, [C.c_void_p, _dp]),
     "lgt_download": (C.c_int, [C.c_void_p, _dp]),
+    "lgt_write_config": (C.c_int, [C.c_void_p, C.c_char_p]),
+    "lgt_read_config": (C.c_int, [C.c_void_p, C.c_char_p, _dp, _dp]),
     "lgt_update": (C.c_int, [C.c_void_p, C.c_int, C.c_int, C.c_double, _dp]),
     "lgt_heatbath_subsweep": (C.c_int, [C.c_void_p, C.c_int, C.c_int, C.c_double, _dp]),
     "lgt_overrelax_subsweep": (C.c_int, [C.c_void_p, C.c_int, C.c_int]),
@@ -154,6 +156,18 @@ class Engine:
         if U.size != 16 * self.vol:
             raise ValueError("field size mismatch")
         _check(self._lib.lgt_upload(self._h, _ptr(U)), "lgt_upload")
+
+    def write_config(self, path):
+        _check(self._lib.lgt_write_config(self._h, os.fsencode(path)), "lgt_write_config")
+
+    def read_config(self, path):
+        """Loads a reference configuration file; returns (file plaquette,
+        recomputed plaquette).  Raises on a header mismatch or a failed
+        plaquette test (the field then stays loaded)."""
+        p0 = C.c_double(0)
+        p1 = C.c_double(0)
+        _check(self._lib.lgt_read_config(self._h, os.fsencode(path), C.byref(p0), C.byref(p1)), "lgt_read_config")
+        return p0.value, p1.value
 
     def download(self):
         U = np.empty((self.vol, 4, 4), dtype=np.float64)

@@ -258,6 +258,14 @@ class RefLib:
         U = np.ascontiguousarray(U, dtype=np.float64)
         self.lib.ref_set_field(_ptr(U))
 
+    def write_config(self, path):
+        self.lib.ref_write_config(str(path).encode())
+
+    def read_config(self, path):
+        """The reference's readConfig: on a header mismatch or a failed
+        plaquette test it calls error(), which ends the process."""
+        self.lib.ref_read_config(str(path).encode())
+
     def get_field(self):
         U = np.zeros((self.vol, 4, 4))
         self.lib.ref_get_field(_ptr(U))

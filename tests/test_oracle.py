@@ -328,3 +328,32 @@ def test_checkerboard_subsweep_is_order_free(L, mu, p):
     np.testing.assert_array_equal(A, B)
     changed = np.nonzero(np.any(A != U, axis=(1, 2)))[0]
     assert set(changed) == set(sites)
+
+
+# ---------------------------------------------------------------------------
+# configuration file format (config_IO.c:40-117,537-621)
+# ---------------------------------------------------------------------------
+@pytest.mark.skipif(not RefLib.available("a6448"), reason="reference build missing")
+def test_reference_config_file_layout(tmp_path):
+    """The reference's writeConfig output parses with the test reader: header
+    (DIM, SUN, extents t first, plaquette) and links site-major, c0..c3."""
+    import config_format as CF
+    ref = RefLib("a6448")
+    ref.setup(3.09, 5)
+    ref.lib.ref_init_field(1)
+    U = ref.get_field()
+    ref.write_config(tmp_path / "conf")
+    dim, sun, L, plaq, V = CF.read(tmp_path / "conf")
+    assert (dim, sun, L) == (4, 2, (6, 4, 4, 8))
+    assert plaq == ref.lib.ref_plaquette()
+    np.testing.assert_array_equal(V, U)
+    # and the reference reads back what the test writer produced
+    W = U.copy()
+    W[:, :, :] = W[::-1]
+    CF.write(tmp_path / "conf2", L, 0.0, W)
+    ref.set_field(W)
+    p = ref.lib.ref_plaquette()
+    CF.write(tmp_path / "conf2", L, p, W)
+    ref.lib.ref_init_field(0)
+    ref.read_config(tmp_path / "conf2")
+    np.testing.assert_array_equal(ref.get_field(), W)
