@@ -151,16 +151,33 @@ def time_vjp(lgt, L, prec, seed, beta, nsteps=400, sp=0.25, R=5, T=5, therm=50):
     return out
 
 
-def time_observables(e):
-    out = {}
-    t0 = time.perf_counter()
+# quaternion product = 16 multiplications + 12 additions
+FLOP_QMUL = 28
+# improved clover charge: ~576 products per site plus the field-strength sums
+FLOP_QTOP_SITE = 16100
+
+
+def time_observables(e, L):
+    """Wilson loops R 1..10 x T 1..16 and the improved charge, with their rate
+    in the reference's operation count: a loop costs 3 directions x 2(R+T)
+    products per site (wilson.c:21-56), the charge ~16 100 flops per site
+    (topologicalcharge.c:112-199); the fused loop kernel performs fewer."""
+    vol = int(np.prod(L))
     R, T = np.meshgrid(np.arange(1, 11), np.arange(1, 17), indexing="ij")
-    e.wilson_loops(R.ravel(), T.ravel())
-    out["wloops_R1-10_T1-16_s"] = time.perf_counter() - t0
+    R, T = R.ravel(), T.ravel()
+    e.sync()
+    t0 = time.perf_counter()
+    e.wilson_loops(R, T)
+    tw = time.perf_counter() - t0
     t0 = time.perf_counter()
     e.qtop(0)
-    out["qtop_improved_s"] = time.perf_counter() - t0
-    return out
+    tq = time.perf_counter() - t0
+    fw = float(vol * 3 * np.sum(2 * (R + T)) * FLOP_QMUL)
+    fq = float(vol * FLOP_QTOP_SITE)
+    return {
+        "wloops_R1-10_T1-16": {"s": tw, "reference_flops": fw, "TFLOPs": fw / tw / 1e12},
+        "qtop_improved": {"s": tq, "reference_flops": fq, "TFLOPs": fq / tq / 1e12},
+    }
 
 
 # one reference sweep in a fresh interpreter (no GPU): prints its own duration
@@ -283,13 +300,13 @@ def main():
     log(f"[bench] fp32 streaming pass (project): {stream32['achieved_GBs']:.0f} GB/s")
     ape32 = time_ape(e32, 1, L)
     log(f"[bench] fp32 spatial APE: {ape32['steps_per_s']:.1f} steps/s")
-    obs32 = time_observables(e32)
+    obs32 = time_observables(e32, L)
     e32.close()
 
     e64, r64 = run_chain(lgt, L, lgt.FP64, args.seed, args.beta, args.n_or, args.warmup, args.steps, args.nplaq)
     log(f"[bench] fp64 sweep: {r64['ms_per_step']:.3f} ms/step, {r64['link_updates_per_s']:.4g} link updates/s")
     ape64 = time_ape(e64, 0, L)
-    obs64 = time_observables(e64)
+    obs64 = time_observables(e64, L)
     log("[bench] APE-400 adjoint (fp64) ...")
     vjp64 = time_vjp(lgt, L, lgt.FP64, args.seed, args.beta, nsteps=args.vjp_steps)
     log(f"[bench] fp64 VJP/forward = {vjp64['vjp_over_forward']:.2f} (forward {vjp64['forward_s']:.3f} s)")
