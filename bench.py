@@ -243,6 +243,59 @@ def cpu_baseline(e64, L, beta, procs):
     return out
 
 
+def cpu_table(beta=3.09):
+    """The reference's own functions (oracle/_ref, compiled from its sources)
+    and the C restatement of them (oracle/su2_oracle.c) on one core: heatbath
+    sweep at 24^4 and 48^4, one spatial APE step, Wilson loops R,T in 1..6 and
+    the improved charge at 24^4.  The reference sweeps lexicographically with
+    RANLUX (update.c:102-176); the restatement runs the engine's checkerboard
+    sweep with Philox (the same per-link work).  No GPU is used."""
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    from oracle_lib import Oracle, RefLib
+    R6, T6 = np.meshgrid(np.arange(1, 7), np.arange(1, 7), indexing="ij")
+    R6, T6 = R6.ravel().astype(np.int32), T6.ravel().astype(np.int32)
+    rows = []
+
+    def timed(f):
+        t0 = time.perf_counter()
+        f()
+        return time.perf_counter() - t0
+
+    for name, L in (("l24", (24, 24, 24, 24)), ("l48", (48, 48, 48, 48))):
+        if not RefLib.available(name):
+            continue
+        vol = int(np.prod(L))
+        o = Oracle(L)
+        U = o.hot(5)
+        o.engine_sweep(U, beta, 0, 5, 0)
+        ref = RefLib(name)
+        ref.setup(beta, 1)
+        ref.set_field(U)
+        n = 2 if L[0] == 24 else 1
+        ref.lib.ref_update(1)
+        t_ref = timed(lambda: ref.lib.ref_update(n)) / n
+        V = U.copy()
+        t_or = timed(lambda: [o.engine_sweep(V, beta, 0, 5, 1 + k) for k in range(n)]) / n
+        rows.append({"op": "heatbath sweep", "lattice": L, "reference_s": t_ref, "restatement_s": t_or,
+                     "reference_link_updates_per_s": 4 * vol / t_ref})
+        ref.set_field(U)
+        V = U.copy()
+        rows.append({"op": "spatial APE step", "lattice": L, "reference_s": timed(lambda: ref.lib.ref_ape(0, 1, 0.25)),
+                     "restatement_s": timed(lambda: o.ape(V, 0, 1, 0.25))})
+        if L[0] == 24:
+            ref.set_field(U)
+            rows.append({"op": "Wilson loops R,T 1..6", "lattice": L,
+                         "reference_s": timed(lambda: ref.wilson(R6, T6)),
+                         "restatement_s": timed(lambda: o.wilson(U, R6, T6, seq=True))})
+            rows.append({"op": "improved charge", "lattice": L,
+                         "reference_s": timed(lambda: ref.lib.ref_qtop_improved()),
+                         "restatement_s": timed(lambda: o.qtop(U, seq=True))})
+        log(f"[cpu-table] {name} done")
+    for r in rows:
+        r["restatement_over_reference"] = r["restatement_s"] / r["reference_s"]
+    return {"cpu_table": rows, "cores": 1}
+
+
 def time_host_transfer(e64):
     """Host <-> device round trip of the FP64 field in the reference's layout
     (what the drop-in entry points pay per call when the field lives in the
@@ -278,6 +331,8 @@ def main():
     ap.add_argument("--seed", type=int, default=7)
     ap.add_argument("--nplaq", type=int, default=50)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--cpu-table", action="store_true",
+                    help="only time the reference and the restatement on the CPU (no GPU) and exit")
     ap.add_argument("--cpu-procs", type=int, default=16,
                     help="independent reference processes for the all-cores CPU figure (1: skip)")
     ap.add_argument("--vjp-steps", type=int, default=400)
@@ -286,6 +341,10 @@ def main():
         print(f"bench.py runs on exactly one GPU; --gpus {args.gpus} is not supported (the engine has no "
               f"multi-GPU path)", file=sys.stderr)
         sys.exit(2)
+
+    if args.cpu_table:
+        print(json.dumps(cpu_table(args.beta)), flush=True)
+        return
 
     import masterarbeit_amd as lgt
     lgt.load_library()
