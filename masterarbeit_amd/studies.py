@@ -54,7 +54,8 @@ def tau_int(x):
         return float(np.mean((a - a.mean()) * (b - b.mean())))
 
     g0 = gamma(0)
-    if g0 <= 0.0:
+    # a constant series (up to the rounding of its mean) has no fluctuation
+    if g0 <= 1e-24 * float(np.mean(x * x)):
         return 0.5, 0.0
     t = 0.5 * g0
     for k in range(1, n):
