@@ -123,6 +123,13 @@ int lgt_download(lgt_engine *e, double *aos);
  * return the two values. */
 int lgt_write_config(lgt_engine *e, const char *path);
 int lgt_read_config(lgt_engine *e, const char *path, double *plaq_file, double *plaq_calc);
+/* readConfigForThermalisation (LGT_Code/modules/io/config_IO.c:348-535): loads
+ * a configuration file of half the engine's extents into the 16 sublattices
+ * of the engine's lattice, mirrored as the reference does (in a mirrored
+ * direction j a link along j moves from x to L_j - 1 - x and is daggered, a
+ * link across j from x to L_j - x, x = 0 to L_j / 2), then the plaquette test
+ * of lgt_read_config against the file's plaquette.  Extents must be even. */
+int lgt_read_config_doubled(lgt_engine *e, const char *path, double *plaq_file, double *plaq_calc);
 
 /* ---- update -------------------------------------------------------------- */
 

@@ -561,22 +561,45 @@ int lgt_write_config(lgt_engine* e, const char* path)
     return LGT_OK;
 }
 
+namespace {
+// header of a configuration file; checks DIM, SUN and the expected extents
+int read_config_header(FILE* f, const char* path, const int expect[4], double* plaq0, const char* who)
+{
+    int32_t info[2], ls[4];
+    if (fread(info, sizeof(int32_t), 2, f) != 2 || fread(ls, sizeof(int32_t), 4, f) != 4 ||
+        fread(plaq0, sizeof(double), 1, f) != 1)
+        return fail(LGT_EINVAL, "%s: read error on %s", who, path);
+    if (info[0] != 4 || info[1] != 2) return fail(LGT_EINVAL, "%s: DIM %d SUN %d, expected 4 2", who, info[0], info[1]);
+    for (int d = 0; d < 4; d++)
+        if (ls[d] != expect[d])
+            return fail(LGT_EINVAL, "%s: lattice %dx%dx%dx%d, expected %dx%dx%dx%d", who, ls[0], ls[1], ls[2], ls[3],
+                        expect[0], expect[1], expect[2], expect[3]);
+    return LGT_OK;
+}
+
+// uploads, recomputes the plaquette and applies readConfig's test (config_IO.c:609-614)
+int load_and_check(lgt_engine* e, const double* aos, double plaq0, double* plaq_file, double* plaq_calc, const char* who)
+{
+    TRY(lgt_upload(e, aos));
+    double plaq = 0.0;
+    TRY(lgt_plaquette(e, &plaq));
+    if (plaq_file) *plaq_file = plaq0;
+    if (plaq_calc) *plaq_calc = plaq;
+    const double eps = std::sqrt(6.0 * (double)e->vol) * (e->prec == LGT_FP64 ? DBL_EPSILON : (double)FLT_EPSILON);
+    if (!(std::fabs(plaq - plaq0) <= eps))
+        return fail(LGT_EINVAL, "%s: plaquette test failed (file %.16g, field %.16g)", who, plaq0, plaq);
+    return LGT_OK;
+}
+}  // namespace
+
 int lgt_read_config(lgt_engine* e, const char* path, double* plaq_file, double* plaq_calc)
 {
     if (!e || !path) return fail(LGT_EINVAL, "lgt_read_config: null argument");
     if (!host_little_endian()) return fail(LGT_EINVAL, "lgt_read_config: big-endian host");
     FileCloser fc{fopen(path, "rb")};
     if (!fc.f) return fail(LGT_EINVAL, "lgt_read_config: cannot open %s", path);
-    int32_t info[2], ls[4];
     double plaq0 = 0.0;
-    if (fread(info, sizeof(int32_t), 2, fc.f) != 2 || fread(ls, sizeof(int32_t), 4, fc.f) != 4 ||
-        fread(&plaq0, sizeof(double), 1, fc.f) != 1)
-        return fail(LGT_EINVAL, "lgt_read_config: read error on %s", path);
-    if (info[0] != 4 || info[1] != 2) return fail(LGT_EINVAL, "lgt_read_config: DIM %d SUN %d, expected 4 2", info[0], info[1]);
-    for (int d = 0; d < 4; d++)
-        if (ls[d] != e->L[d])
-            return fail(LGT_EINVAL, "lgt_read_config: lattice %dx%dx%dx%d, engine %dx%dx%dx%d", ls[0], ls[1], ls[2], ls[3],
-                        e->L[0], e->L[1], e->L[2], e->L[3]);
+    TRY(read_config_header(fc.f, path, e->L, &plaq0, "lgt_read_config"));
     std::vector<double> aos;
     try {
         aos.resize(16 * (size_t)e->vol);
@@ -585,15 +608,59 @@ int lgt_read_config(lgt_engine* e, const char* path, double* plaq_file, double* 
     }
     if (fread(aos.data(), sizeof(double), aos.size(), fc.f) != aos.size())
         return fail(LGT_EINVAL, "lgt_read_config: %s is short", path);
-    TRY(lgt_upload(e, aos.data()));
-    double plaq = 0.0;
-    TRY(lgt_plaquette(e, &plaq));
-    if (plaq_file) *plaq_file = plaq0;
-    if (plaq_calc) *plaq_calc = plaq;
-    const double eps = std::sqrt(6.0 * (double)e->vol) * (e->prec == LGT_FP64 ? DBL_EPSILON : (double)FLT_EPSILON);
-    if (!(std::fabs(plaq - plaq0) <= eps))
-        return fail(LGT_EINVAL, "lgt_read_config: plaquette test failed (file %.16g, field %.16g)", plaq0, plaq);
-    return LGT_OK;
+    return load_and_check(e, aos.data(), plaq0, plaq_file, plaq_calc, "lgt_read_config");
+}
+
+int lgt_read_config_doubled(lgt_engine* e, const char* path, double* plaq_file, double* plaq_calc)
+{
+    if (!e || !path) return fail(LGT_EINVAL, "lgt_read_config_doubled: null argument");
+    if (!host_little_endian()) return fail(LGT_EINVAL, "lgt_read_config_doubled: big-endian host");
+    int half[4];
+    for (int d = 0; d < 4; d++) {
+        if (e->L[d] % 2) return fail(LGT_EINVAL, "lgt_read_config_doubled: odd extent %d", e->L[d]);
+        half[d] = e->L[d] / 2;
+    }
+    FileCloser fc{fopen(path, "rb")};
+    if (!fc.f) return fail(LGT_EINVAL, "lgt_read_config_doubled: cannot open %s", path);
+    double plaq0 = 0.0;
+    TRY(read_config_header(fc.f, path, half, &plaq0, "lgt_read_config_doubled"));
+    const int64_t vs = e->vol / 16;
+    std::vector<double> aos, small;
+    try {
+        aos.resize(16 * (size_t)e->vol);
+        small.resize(16 * (size_t)vs);
+    } catch (const std::bad_alloc&) {
+        return fail(LGT_ENOMEM, "lgt_read_config_doubled: host buffer");
+    }
+    if (fread(small.data(), sizeof(double), small.size(), fc.f) != small.size())
+        return fail(LGT_EINVAL, "lgt_read_config_doubled: %s is short", path);
+    const int* L = e->L;
+    for (int64_t n = 0; n < vs; n++) {
+        int c[4];
+        int64_t r = n;
+        for (int d = 3; d >= 0; d--) {
+            c[d] = (int)(r % half[d]);
+            r /= half[d];
+        }
+        for (int mu = 0; mu < 4; mu++) {
+            const double* u = &small[16 * n + 4 * mu];
+            for (int k = 0; k < 16; k++) {   // sublattice k: bit j mirrors direction j
+                int64_t idx = 0;
+                for (int j = 0; j < 4; j++) {
+                    int x = c[j];
+                    if ((k >> j) & 1) x = (j == mu) ? L[j] - c[j] - 1 : (c[j] == 0 ? half[j] : L[j] - c[j]);
+                    idx = idx * L[j] + x;
+                }
+                double* o = &aos[16 * idx + 4 * mu];
+                const double sg = ((k >> mu) & 1) ? -1.0 : 1.0;   // mirrored along the link: dagger
+                o[0] = u[0];
+                o[1] = sg * u[1];
+                o[2] = sg * u[2];
+                o[3] = sg * u[3];
+            }
+        }
+    }
+    return load_and_check(e, aos.data(), plaq0, plaq_file, plaq_calc, "lgt_read_config_doubled");
 }
 
 int lgt_heatbath_subsweep(lgt_engine* e, int mu, int parity, double beta, double* acc_sum)

@@ -42,6 +42,7 @@ SIGNATURES = {
     "lgt_download": (C.c_int, [C.c_void_p, _dp]),
     "lgt_write_config": (C.c_int, [C.c_void_p, C.c_char_p]),
     "lgt_read_config": (C.c_int, [C.c_void_p, C.c_char_p, _dp, _dp]),
+    "lgt_read_config_doubled": (C.c_int, [C.c_void_p, C.c_char_p, _dp, _dp]),
     "lgt_update": (C.c_int, [C.c_void_p, C.c_int, C.c_int, C.c_double, _dp]),
     "lgt_heatbath_subsweep": (C.c_int, [C.c_void_p, C.c_int, C.c_int, C.c_double, _dp]),
     "lgt_overrelax_subsweep": (C.c_int, [C.c_void_p, C.c_int, C.c_int]),
@@ -167,6 +168,15 @@ class Engine:
         p0 = C.c_double(0)
         p1 = C.c_double(0)
         _check(self._lib.lgt_read_config(self._h, os.fsencode(path), C.byref(p0), C.byref(p1)), "lgt_read_config")
+        return p0.value, p1.value
+
+    def read_config_doubled(self, path):
+        """Loads a configuration of half the extents, doubled by reflection
+        (readConfigForThermalisation); returns (file, recomputed) plaquette."""
+        p0 = C.c_double(0)
+        p1 = C.c_double(0)
+        _check(self._lib.lgt_read_config_doubled(self._h, os.fsencode(path), C.byref(p0), C.byref(p1)),
+               "lgt_read_config_doubled")
         return p0.value, p1.value
 
     def download(self):

@@ -128,3 +128,4 @@ double ref_qtop_clover(void)
 
 void ref_write_config(char *path) { writeConfig(path); }
 void ref_read_config(char *path) { readConfig(path); }
+void ref_read_config_doubled(char *path) { readConfigForThermalisation(path); }

@@ -243,6 +243,7 @@ class RefLib:
                 "ref_qtop_clover": (C.c_double, []),
                 "ref_write_config": (None, [C.c_char_p]),
                 "ref_read_config": (None, [C.c_char_p]),
+                "ref_read_config_doubled": (None, [C.c_char_p]),
             }
             for fname, (res, args) in sig.items():
                 f = getattr(lib, fname)
@@ -265,6 +266,11 @@ class RefLib:
         """The reference's readConfig: on a header mismatch or a failed
         plaquette test it calls error(), which ends the process."""
         self.lib.ref_read_config(str(path).encode())
+
+    def read_config_doubled(self, path):
+        """readConfigForThermalisation: a configuration of half the extents,
+        doubled by reflection; exits the process on a failed check."""
+        self.lib.ref_read_config_doubled(str(path).encode())
 
     def get_field(self):
         U = np.zeros((self.vol, 4, 4))

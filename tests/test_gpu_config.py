@@ -99,3 +99,46 @@ def test_read_config_rejects_mismatch(lgt, tmp_path):
         assert e.read_config(tmp_path / "good") == (1.0, 1.0)
         with pytest.raises(lgt.LgtError):
             e.read_config(tmp_path / "missing")
+
+
+@pytest.mark.parametrize("small,big", [("l4", "l8"), ("l12", "l24")])
+def test_reflection_doubling_matches_reference(lgt, tmp_path, small, big):
+    """lgt_read_config_doubled against the reference's
+    readConfigForThermalisation (config_IO.c:348-535) on the same file: the
+    doubled fields are identical and both pass the plaquette test."""
+    if not (RefLib.available(small) and RefLib.available(big)):
+        pytest.skip("reference build missing")
+    ref = RefLib(small)
+    ref.setup(3.09, 4)
+    ref.lib.ref_init_field(1)
+    ref.lib.ref_update(3)
+    ref.write_config(tmp_path / "small")
+    code = (
+        "import sys, numpy as np\n"
+        f"sys.path.insert(0, {os.path.dirname(__file__)!r})\n"
+        "from oracle_lib import RefLib\n"
+        f"r = RefLib({big!r}); r.setup(3.09, 1); r.lib.ref_init_field(0)\n"
+        f"r.read_config_doubled({str(tmp_path / 'small')!r})\n"
+        f"np.save({str(tmp_path / 'big.npy')!r}, r.get_field())\n"
+        "print('ok')\n")
+    res = subprocess.run([sys.executable, "-c", code], capture_output=True, text=True, timeout=120)
+    assert res.returncode == 0 and "ok" in res.stdout, res.stdout + res.stderr
+    Lbig = RefLib(big).L
+    with lgt.Engine(Lbig, 0, 1) as e:
+        p0, p1 = e.read_config_doubled(tmp_path / "small")
+        V = e.download()
+    np.testing.assert_array_equal(V, np.load(tmp_path / "big.npy"))
+    assert p0 == ref.lib.ref_plaquette()
+    assert abs(p1 - p0) <= np.sqrt(6 * np.prod(Lbig)) * np.finfo(np.float64).eps
+
+
+def test_reflection_doubling_rejects_wrong_size(lgt, tmp_path):
+    U = np.zeros((4 ** 4, 4, 4))
+    U[:, :, 0] = 1.0
+    CF.write(tmp_path / "c4", (4, 4, 4, 4), 1.0, U)
+    with lgt.Engine((6, 8, 8, 8), 0, 1) as e:
+        with pytest.raises(lgt.LgtError):
+            e.read_config_doubled(tmp_path / "c4")
+    with lgt.Engine((8, 8, 8, 8), 0, 1) as e:
+        assert e.read_config_doubled(tmp_path / "c4") == (1.0, 1.0)
+        assert np.all(e.download()[:, :, 0] == 1.0)
