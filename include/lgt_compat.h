@@ -85,8 +85,9 @@ extern double getTime(void);                              /* utils.c:56-78 */
 extern void ranlxd(double r[], int n);                    /* ranlxd.c:571-586 */
 
 /* ---- exported entry points (LGT_Code/include/modules.h) ---- */
-/* modules.h:99, update.c:102-176: utype must be 1 (heatbath); the GPU sweep is
- * checkerboard whatever stype says. */
+/* modules.h:99, update.c:102-176: utype 1 = heatbath (+ LGT_N_OR overrelaxation
+ * sweeps), 0 = Metropolis with runParams.eps; the GPU sweep is checkerboard
+ * whatever stype says. */
 void gaugefieldUpdate(int iup, int nup, int utype, int stype);
 double plaquette(void);                                   /* modules.h:27, plaquette.c:67-101 */
 double gaugeAction(void);                                 /* modules.h:28, plaquette.c:104-110 */

@@ -73,7 +73,8 @@ typedef struct lgt_engine lgt_engine;
 #define LGT_K_APE_SAVE 9   /* smearing steps recomputed by the adjoint */
 #define LGT_K_VJP 10       /* adjoint steps of the smearing */
 #define LGT_K_WLOOP_GRAD 11
-#define LGT_K_COUNT 12
+#define LGT_K_METROPOLIS 12
+#define LGT_K_COUNT 13
 
 /* ---- life cycle ---------------------------------------------------------- */
 
@@ -144,6 +145,14 @@ int lgt_read_config_doubled(lgt_engine *e, const char *path, double *plaq_file, 
  * acceptance (nullable) = mean over links and sweeps of 1/trials, as the
  * reference logs it (update.c:159,170-173). */
 int lgt_update(lgt_engine *e, int nsweeps, int n_or, double beta, double *acceptance);
+/* gaugefieldUpdate(iup, nup, utype=0, stype) (update.c:102-176 with
+ * localMetropolisUpdate, LGT_Code/modules/update/metro.c:65-109): nsweeps
+ * checkerboard sweeps (sub-sweeps mu = 0..3, parity 0,1) of one Metropolis hit
+ * per link with the proposal expx(eps, X) U, X_i = 1 - 2 r_i (exp_fct.c:137-151;
+ * eps = runParams.eps), then one projection of the field (update.c:166).
+ * Uniforms: Philox4x32-10, counter (cb site, mu|parity<<2|2<<3, sweep index,
+ * draw), 4 per link.  acceptance (nullable) = accepted fraction. */
+int lgt_metropolis_update(lgt_engine *e, int nsweeps, double beta, double eps, double *acceptance);
 /* single sub-sweeps (no projection), for replays of a sweep */
 int lgt_heatbath_subsweep(lgt_engine *e, int mu, int parity, double beta, double *acc_sum);
 int lgt_overrelax_subsweep(lgt_engine *e, int mu, int parity);
@@ -216,6 +225,11 @@ int lgt_heatbath_harness(int n, const double *staples, double beta, const double
 /* The overrelaxation map on given staples and links (no reference
  * counterpart; the update above adds it to the heatbath). */
 int lgt_overrelax_harness(int n, const double *staples, const double *links_in, double *links_out);
+/* One Metropolis hit (metro.c:80-105 after the staple) on given staples and
+ * links with the uniforms uniforms[4i..4i+3] (3 for the proposal, 1 for the
+ * acceptance, the reference's order); accepted[i] = 0 or 1. */
+int lgt_metropolis_harness(int n, const double *staples, const double *links_in, double beta, double eps,
+                           const double *uniforms, double *links_out, int *accepted);
 
 /* ---- timing --------------------------------------------------------------- */
 /* With LGT_OPT_PROFILE on, every launch is bracketed by HIP events on the

@@ -438,6 +438,81 @@ __global__ void __launch_bounds__(BLOCK) k_overrelax(Planes<T> P, Geo g, int p)
 }
 
 // ---------------------------------------------------------------------------
+// Metropolis, one hit (LGT_Code/modules/update/metro.c:65-109 with the proposal
+// expx of exp_fct.c:137-151): X_i = 1 - 2 r_i, U' = expx(eps, X) U, accepted
+// iff r_4 <= exp(beta (Re tr U'A - Re tr UA) / 2).  The FP64 form keeps the
+// reference's operation order; returns 1 if accepted.
+// ---------------------------------------------------------------------------
+template <typename T, class Src>
+__device__ __forceinline__ int metropolis_link(const Q<T>& A, T beta, T eps, Src& src, const Q<T>& U, Q<T>& Un)
+{
+    T lpl_old = T(2) * qmul(U, A).c0;
+    T X1 = (T(1) - T(2) * (T)src.next());
+    T X2 = (T(1) - T(2) * (T)src.next());
+    T X3 = (T(1) - T(2) * (T)src.next());
+    T zw = eps * eps * (X1 * X1 + X2 * X2 + X3 * X3) / T(4);
+    Q<T> etx;
+    etx.c0 = T(1) - zw;
+    zw = T(1) / (T(1) + zw);
+    etx.c0 *= zw;
+    etx.c1 = zw * eps * X1;
+    etx.c2 = zw * eps * X2;
+    etx.c3 = zw * eps * X3;
+    Q<T> upr = qmul(etx, U);
+    T lpl = T(2) * qmul(upr, A).c0;
+    T dact = exp(beta * (lpl - lpl_old) / T(2));
+    T r = (T)src.next();
+    const int acc = (r <= dact) ? 1 : 0;
+    Un = acc ? upr : U;
+    return acc;
+}
+
+// FP32 uniforms: one Philox call gives the hit's four 24-bit uniforms
+struct RngSrc32 {
+    U32x4 w;
+    int k;
+    __device__ __forceinline__ RngSrc32(uint64_t seed, uint32_t site, uint32_t tag, uint32_t sweep) : k(0)
+    {
+        LinkRng rng(seed, site, tag, sweep);
+        w = rng.next();
+    }
+    __device__ __forceinline__ float next()
+    {
+        uint32_t v = (k == 0) ? w.x : (k == 1) ? w.y : (k == 2) ? w.z : w.w;
+        k++;
+        return u24(v);
+    }
+};
+
+template <typename T> struct MetroSrc;
+template <> struct MetroSrc<double> { typedef RngSrc64 type; };
+template <> struct MetroSrc<float> { typedef RngSrc32 type; };
+
+// Metropolis sub-sweep of direction MU, parity p; acc_partial[tile] receives
+// the block's number of accepted links (nullable).
+template <typename T, int MU>
+__global__ void __launch_bounds__(BLOCK) k_metropolis(Planes<T> P, Geo g, int p, T beta, T eps, uint64_t seed,
+                                                      uint32_t sweep, DD* __restrict__ acc_partial)
+{
+    uint32_t base = tile_base(g);
+    uint32_t tile = base / BLOCK;
+    uint32_t i = base + threadIdx.x;
+    int acc = 0;
+    if (i < g.vh) {
+        Site s = cb_site(g, i, p);
+        Q<T> A = update_staple<T, MU>(P, g, s, p);
+        Q<T> U = qload<T>(P.p[MU][p], i), Un;
+        typename MetroSrc<T>::type src(seed, i, (uint32_t)(MU | (p << 2) | (KIND_METRO << 3)), sweep);
+        acc = metropolis_link<T>(A, beta, eps, src, U, Un);
+        qstore<T>(P.p[MU][p], i, Un);
+    }
+    if (acc_partial) {
+        DD s = block_sum_dd(DD{(double)acc, 0.0});
+        if (threadIdx.x == 0) acc_partial[tile] = s;
+    }
+}
+
+// ---------------------------------------------------------------------------
 // per-link harnesses with injected inputs (FP64)
 // ---------------------------------------------------------------------------
 static __global__ void __launch_bounds__(BLOCK) k_hb_harness(int n, const double* __restrict__ staples, double beta,
@@ -453,6 +528,20 @@ static __global__ void __launch_bounds__(BLOCK) k_hb_harness(int n, const double
     int c = heatbath_link64(A, beta, src, U);
     links[4 * i] = U.c0; links[4 * i + 1] = U.c1; links[4 * i + 2] = U.c2; links[4 * i + 3] = U.c3;
     counts[i] = (src.pos > src.n) ? -1 : c;
+}
+
+static __global__ void __launch_bounds__(BLOCK) k_metro_harness(int n, const double* __restrict__ staples,
+                                                         const double* __restrict__ links_in, double beta, double eps,
+                                                         const double* __restrict__ uniforms,
+                                                         double* __restrict__ links, int* __restrict__ accepted)
+{
+    int i = blockIdx.x * BLOCK + threadIdx.x;
+    if (i >= n) return;
+    BufSrc src{uniforms + 4 * (size_t)i, 4, 0};
+    Q<double> A{staples[4 * i], staples[4 * i + 1], staples[4 * i + 2], staples[4 * i + 3]};
+    Q<double> U{links_in[4 * i], links_in[4 * i + 1], links_in[4 * i + 2], links_in[4 * i + 3]}, R;
+    accepted[i] = metropolis_link<double>(A, beta, eps, src, U, R);
+    links[4 * i] = R.c0; links[4 * i + 1] = R.c1; links[4 * i + 2] = R.c2; links[4 * i + 3] = R.c3;
 }
 
 static __global__ void __launch_bounds__(BLOCK) k_or_harness(int n, const double* __restrict__ staples,

@@ -30,6 +30,9 @@ template <typename T> struct Launch {
     static hipError_t heatbath(hipStream_t s, const Geo& g, const RawPlanes& P, int mu, int parity, double beta,
                                uint64_t seed, uint32_t sweep, void* acc_partial);
     static hipError_t overrelax(hipStream_t s, const Geo& g, const RawPlanes& P, int mu, int parity);
+    // acc_partial: nullable, receives ceil(vh/256) block sums of accepted links
+    static hipError_t metropolis(hipStream_t s, const Geo& g, const RawPlanes& P, int mu, int parity, double beta,
+                                 double eps, uint64_t seed, uint32_t sweep, void* acc_partial);
     // partial: 2*ceil(vh/256) double-doubles
     static hipError_t plaquette(hipStream_t s, const Geo& g, const RawPlanes& P, void* partial);
     static hipError_t ape(hipStream_t s, const Geo& g, const RawPlanes& in, const RawPlanes& out, int mode, double sp);
@@ -54,6 +57,8 @@ hipError_t launch_reduce_rows(hipStream_t s, const void* in, int nrows, int ncol
 hipError_t launch_hb_harness(hipStream_t s, int n, const double* staples, double beta, const double* uniforms,
                              const int* offsets, const int* lens, double* links, int* counts);
 hipError_t launch_or_harness(hipStream_t s, int n, const double* staples, const double* links_in, double* links);
+hipError_t launch_metro_harness(hipStream_t s, int n, const double* staples, const double* links_in, double beta,
+                                double eps, const double* uniforms, double* links, int* accepted);
 
 inline uint32_t grid_x(const Geo& g) { return (g.vh + 255u) / 256u; }
 

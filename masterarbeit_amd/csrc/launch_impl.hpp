@@ -82,6 +82,23 @@ hipError_t Launch<T>::overrelax(hipStream_t s, const Geo& g, const RawPlanes& P,
 }
 
 template <typename T>
+hipError_t Launch<T>::metropolis(hipStream_t s, const Geo& g, const RawPlanes& P, int mu, int parity, double beta,
+                                 double eps, uint64_t seed, uint32_t sweep, void* acc_partial)
+{
+    dim3 grid(grid_x(g)), block(BLOCK);
+    Planes<T> Pl = planes<T>(P);
+    DD* ap = (DD*)acc_partial;
+    const T b = (T)beta, e = (T)eps;
+    switch (mu) {
+    case 0: hipLaunchKernelGGL((k_metropolis<T, 0>), grid, block, 0, s, Pl, g, parity, b, e, seed, sweep, ap); break;
+    case 1: hipLaunchKernelGGL((k_metropolis<T, 1>), grid, block, 0, s, Pl, g, parity, b, e, seed, sweep, ap); break;
+    case 2: hipLaunchKernelGGL((k_metropolis<T, 2>), grid, block, 0, s, Pl, g, parity, b, e, seed, sweep, ap); break;
+    default: hipLaunchKernelGGL((k_metropolis<T, 3>), grid, block, 0, s, Pl, g, parity, b, e, seed, sweep, ap); break;
+    }
+    return hipGetLastError();
+}
+
+template <typename T>
 hipError_t Launch<T>::plaquette(hipStream_t s, const Geo& g, const RawPlanes& P, void* partial)
 {
     hipLaunchKernelGGL(k_plaquette<T>, dim3(2 * grid_x(g)), dim3(BLOCK), 0, s, planes<T>(P), g, (DD*)partial);

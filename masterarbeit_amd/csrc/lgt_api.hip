@@ -663,6 +663,37 @@ int lgt_read_config_doubled(lgt_engine* e, const char* path, double* plaq_file, 
     return load_and_check(e, aos.data(), plaq0, plaq_file, plaq_calc, "lgt_read_config_doubled");
 }
 
+int lgt_metropolis_update(lgt_engine* e, int nsweeps, double beta, double eps, double* acceptance)
+{
+    if (!e || nsweeps < 0) return fail(LGT_EINVAL, "lgt_metropolis_update: nsweeps must be >= 0");
+    if (!(beta >= 0.0) || !std::isfinite(beta) || !std::isfinite(eps))
+        return fail(LGT_EINVAL, "lgt_metropolis_update: beta = %g, eps = %g", beta, eps);
+    TRY(set_dev(e));
+    const uint32_t nb = nbx(e);
+    double accepted = 0.0;
+    if (acceptance) TRY(ensure(e, &e->partial, &e->partial_bytes, sizeof(double) * 2 * 8 * (size_t)nb, "partials"));
+    for (int s = 0; s < nsweeps; s++) {
+        for (int mu = 0; mu < 4; mu++)
+            for (int p = 0; p < 2; p++) {
+                void* ap = acceptance ? (char*)e->partial + sizeof(double) * 2 * (size_t)nb * (mu * 2 + p) : nullptr;
+                TRY(run(e, LGT_K_METROPOLIS, "metropolis", [&] {
+                    return DISPATCH(e, metropolis(e->stream, kgeo(e, LGT_K_METROPOLIS, mu), e->cur, mu, p, beta, eps,
+                                                  e->seed, e->sweep, ap));
+                }));
+            }
+        if (acceptance) {
+            double a = 0.0;
+            TRY(reduce_to_host(e, 1, (int)(8 * nb), &a));
+            accepted += a;
+        }
+        e->sweep++;
+    }
+    TRY(run(e, LGT_K_PROJECT, "project", [&] { return DISPATCH(e, project(e->stream, e->geo, e->cur)); }));
+    if (acceptance) *acceptance = nsweeps > 0 ? accepted / ((double)(4 * e->vol) * nsweeps) : 0.0;
+    HIP_TRY(hipStreamSynchronize(e->stream), "hipStreamSynchronize");
+    return LGT_OK;
+}
+
 int lgt_heatbath_subsweep(lgt_engine* e, int mu, int parity, double beta, double* acc_sum)
 {
     if (!e || mu < 0 || mu > 3 || parity < 0 || parity > 1) return fail(LGT_EINVAL, "lgt_heatbath_subsweep: mu/parity");
@@ -1096,6 +1127,33 @@ int lgt_overrelax_harness(int n, const double* staples, const double* links_in, 
         if (err != hipSuccess) rc = hip_fail(err, "lgt_overrelax_harness");
     }
     hipFree(ds); hipFree(di); hipFree(dl);
+    return rc;
+}
+
+int lgt_metropolis_harness(int n, const double* staples, const double* links_in, double beta, double eps,
+                           const double* uniforms, double* links_out, int* accepted)
+{
+    if (n <= 0 || !staples || !links_in || !uniforms || !links_out || !accepted)
+        return fail(LGT_EINVAL, "lgt_metropolis_harness: bad arguments");
+    double *ds = nullptr, *di = nullptr, *du = nullptr, *dl = nullptr;
+    int* da = nullptr;
+    int rc = LGT_OK;
+    hipError_t err = hipSuccess;
+    if ((err = hipMalloc(&ds, sizeof(double) * 4 * n)) != hipSuccess || (err = hipMalloc(&di, sizeof(double) * 4 * n)) != hipSuccess ||
+        (err = hipMalloc(&du, sizeof(double) * 4 * n)) != hipSuccess || (err = hipMalloc(&dl, sizeof(double) * 4 * n)) != hipSuccess ||
+        (err = hipMalloc(&da, sizeof(int) * n)) != hipSuccess)
+        rc = fail(LGT_ENOMEM, "lgt_metropolis_harness: hipMalloc: %s", hipGetErrorString(err));
+    if (rc == LGT_OK) {
+        err = hipMemcpy(ds, staples, sizeof(double) * 4 * n, hipMemcpyHostToDevice);
+        if (err == hipSuccess) err = hipMemcpy(di, links_in, sizeof(double) * 4 * n, hipMemcpyHostToDevice);
+        if (err == hipSuccess) err = hipMemcpy(du, uniforms, sizeof(double) * 4 * n, hipMemcpyHostToDevice);
+        if (err == hipSuccess) err = launch_metro_harness(nullptr, n, ds, di, beta, eps, du, dl, da);
+        if (err == hipSuccess) err = hipDeviceSynchronize();
+        if (err == hipSuccess) err = hipMemcpy(links_out, dl, sizeof(double) * 4 * n, hipMemcpyDeviceToHost);
+        if (err == hipSuccess) err = hipMemcpy(accepted, da, sizeof(int) * n, hipMemcpyDeviceToHost);
+        if (err != hipSuccess) rc = hip_fail(err, "lgt_metropolis_harness");
+    }
+    hipFree(ds); hipFree(di); hipFree(du); hipFree(dl); hipFree(da);
     return rc;
 }
 

@@ -181,14 +181,17 @@ void lgt_compat_transfers(long long *uploads, long long *downloads)
 /* ---- the reference's entry points ---- */
 void gaugefieldUpdate(int iup, int nup, int utype, int stype)
 {
-    char m1[] = "Only the heatbath (SIM_TYPE=1) runs on the GPU engine!";
+    char m1[] = "Unknown update type! Only Metropolis (SIM_TYPE=0) and heatbath (SIM_TYPE=1) implemented!";
     char m2[] = "Unknown sweep type! Only sequential (SWEEP_TYPE=0) and random (SWEEP_TYPE=1) sweeps implemented!";
-    error(utype != 1, NAME, m1);
+    error(utype != 0 && utype != 1, NAME, m1);
     error(stype != 0 && stype != 1, NAME, m2);
     double t1 = getTime();
     pu_on_device();
     double acc = 0.0;
-    check(lgt_update(g_eng, nup, g_nor, runParams.beta, &acc), "lgt_update");
+    if (utype == 1)
+        check(lgt_update(g_eng, nup, g_nor, runParams.beta, &acc), "lgt_update");
+    else
+        check(lgt_metropolis_update(g_eng, nup, runParams.beta, runParams.eps, &acc), "lgt_metropolis_update");
     device_changed();
     double t2 = getTime();
     char fmt[] = "\nUpdate %d done:\n(algorithm %d; sweep %d; acc=%.3f)\nupdate took %.9f seconds\n";

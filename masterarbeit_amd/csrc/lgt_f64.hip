@@ -28,4 +28,12 @@ hipError_t launch_or_harness(hipStream_t s, int n, const double* staples, const 
     return hipGetLastError();
 }
 
+hipError_t launch_metro_harness(hipStream_t s, int n, const double* staples, const double* links_in, double beta,
+                                double eps, const double* uniforms, double* links, int* accepted)
+{
+    hipLaunchKernelGGL(k_metro_harness, dim3((n + BLOCK - 1) / BLOCK), dim3(BLOCK), 0, s, n, staples, links_in, beta,
+                       eps, uniforms, links, accepted);
+    return hipGetLastError();
+}
+
 }  // namespace lgt

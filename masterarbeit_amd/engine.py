@@ -21,9 +21,9 @@ APE_SPATIAL, APE_ALL, APE_TEMPORAL = 0, 1, 2
 Q_IMPROVED, Q_CLOVER = 0, 1
 OPT_TILE_ORDER, OPT_PROFILE, OPT_GROUP = 0, 1, 2
 (K_HEATBATH, K_OVERRELAX, K_PROJECT, K_PLAQUETTE, K_APE, K_WLOOP, K_QTOP, K_REDUCE, K_FIELD, K_APE_SAVE, K_VJP,
- K_WLOOP_GRAD) = range(12)
+ K_WLOOP_GRAD, K_METROPOLIS) = range(13)
 KERNEL_CLASSES = ["heatbath", "overrelax", "project", "plaquette", "ape", "wloop", "qtop", "reduce", "field",
-                  "ape_save", "vjp", "wloop_grad"]
+                  "ape_save", "vjp", "wloop_grad", "metropolis"]
 
 _dp = C.POINTER(C.c_double)
 _ip = C.POINTER(C.c_int)
@@ -60,6 +60,8 @@ SIGNATURES = {
     "lgt_wloop_grad": (C.c_int, [C.c_void_p, C.c_int, C.c_int, _dp]),
     "lgt_heatbath_harness": (C.c_int, [C.c_int, _dp, C.c_double, _dp, _ip, _ip, _dp, _ip]),
     "lgt_overrelax_harness": (C.c_int, [C.c_int, _dp, _dp, _dp]),
+    "lgt_metropolis_update": (C.c_int, [C.c_void_p, C.c_int, C.c_double, C.c_double, _dp]),
+    "lgt_metropolis_harness": (C.c_int, [C.c_int, _dp, _dp, C.c_double, C.c_double, _dp, _dp, _ip]),
     "lgt_profile_read": (C.c_int, [C.c_void_p, C.c_int, _dp, C.POINTER(C.c_int64)]),
     "lgt_profile_reset": (C.c_int, [C.c_void_p]),
     "lgt_sync": (C.c_int, [C.c_void_p]),
@@ -191,6 +193,12 @@ class Engine:
                "lgt_update")
         return acc.value if acceptance else None
 
+    def metropolis_update(self, nsweeps, beta, eps, acceptance=False):
+        acc = C.c_double(0)
+        _check(self._lib.lgt_metropolis_update(self._h, nsweeps, beta, eps, C.byref(acc) if acceptance else None),
+               "lgt_metropolis_update")
+        return acc.value if acceptance else None
+
     def heatbath_subsweep(self, mu, parity, beta, acceptance=False):
         acc = C.c_double(0)
         _check(self._lib.lgt_heatbath_subsweep(self._h, mu, parity, beta, C.byref(acc) if acceptance else None),
@@ -287,3 +295,20 @@ def overrelax_harness(staples, links_in):
     out = np.zeros((n, 4))
     _check(lib.lgt_overrelax_harness(n, _ptr(staples), _ptr(links_in), _ptr(out)), "lgt_overrelax_harness")
     return out
+
+
+def metropolis_harness(staples, links_in, beta, eps, uniforms):
+    """One Metropolis hit per link on the GPU (FP64) with injected uniforms
+    (4 per link); returns (new links, accepted flags)."""
+    lib = load_library()
+    staples = np.ascontiguousarray(staples, dtype=np.float64)
+    links_in = np.ascontiguousarray(links_in, dtype=np.float64)
+    uniforms = np.ascontiguousarray(uniforms, dtype=np.float64)
+    n = staples.shape[0]
+    if links_in.shape != (n, 4) or uniforms.shape != (n, 4):
+        raise ValueError("metropolis_harness: shapes")
+    out = np.zeros((n, 4))
+    acc = np.zeros(n, dtype=np.int32)
+    _check(lib.lgt_metropolis_harness(n, _ptr(staples), _ptr(links_in), beta, eps, _ptr(uniforms), _ptr(out),
+                                      _ptr(acc, C.c_int)), "lgt_metropolis_harness")
+    return out, acc

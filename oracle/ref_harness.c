@@ -91,6 +91,12 @@ void ref_staple(long n, int mu, double *A)
 
 double ref_heatbath_link(long n, int mu) { return localHeatbathUpdate((uint64_t)n, mu, 1); }
 
+double ref_metropolis_link(long n, int mu, double eps)
+{
+    runParams.eps = eps;
+    return localMetropolisUpdate((uint64_t)n, mu, 1);
+}
+
 void ref_update(int nup) { gaugefieldUpdate(1, nup, 1, 0); }
 
 void ref_project(void) { project_gfield_to_sun(pu); }

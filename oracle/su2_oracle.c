@@ -177,7 +177,7 @@ typedef struct {
     int have;    /* uniforms left in w: 0, 1 or 2 */
 } olink_rng_t;
 
-enum { OR_KIND_HB = 0, OR_KIND_HOT = 1 };
+enum { OR_KIND_HB = 0, OR_KIND_HOT = 1, OR_KIND_METRO = 2 };
 
 static void lrng_init(olink_rng_t *r, uint64_t seed, uint32_t site, uint32_t tag, uint32_t sweep)
 {
@@ -416,6 +416,82 @@ double or_engine_sweep(const olat_t *lat, double *U, double beta, int n_or, uint
             for (int p = 0; p < 2; p++) or_or_subsweep(lat, U, mu, p);
     if (n_or == 0) or_project_field(lat, U);
     return acc / (double)(lat->vol * 4);
+}
+
+/* ------------------------------------------------------------------------ */
+/* Metropolis (modules/update/metro.c:65-109, exp_fct.c:137-151)            */
+/* ------------------------------------------------------------------------ */
+static int metropolis_link(const double A[4], double beta, double eps, usrc_t *src, const double U[4],
+                           double Unew[4])
+{
+    double zw4[4], lpl_old, lpl, X[3], etx[4], upr[4];
+    or_qmul(U, A, zw4);                                   /* sun_mul(zw, upr, stap), :87 */
+    lpl_old = 2. * zw4[0];                                /* su2_trace, gauge.h:177-178 */
+    for (int i = 0; i < 3; i++) X[i] = (1. - 2. * usrc_next(src));   /* proposeNewLink, :65-76 */
+    double zw = eps * eps * (X[0] * X[0] + X[1] * X[1] + X[2] * X[2]) / 4.;   /* expx */
+    etx[0] = 1. - zw;
+    zw = 1. / (1. + zw);
+    etx[0] *= zw;
+    etx[1] = zw * eps * X[0];
+    etx[2] = zw * eps * X[1];
+    etx[3] = zw * eps * X[2];
+    or_qmul(etx, U, upr);                                 /* su2_mat_mul(uz, etx, *u) */
+    or_qmul(upr, A, zw4);
+    lpl = 2. * zw4[0];
+    double dact = exp(beta * (lpl - lpl_old) / 2.);       /* :93, SUN = 2 */
+    double r = usrc_next(src);
+    if (r <= dact) {
+        memcpy(Unew, upr, sizeof(upr));
+        return 1;
+    }
+    memcpy(Unew, U, 4 * sizeof(double));
+    return 0;
+}
+
+int or_metropolis_link(const double A[4], double beta, double eps, or_ubuf_t *ub, const double U[4],
+                       double Unew[4])
+{
+    usrc_t s = { ub, NULL };
+    return metropolis_link(A, beta, eps, &s, U, Unew);
+}
+
+int or_metropolis_batch(int64_t n, const double *staples, const double *links_in, double beta, double eps,
+                        const double *uniforms, double *links, int *accepted)
+{
+    for (int64_t i = 0; i < n; i++) {
+        or_ubuf_t ub = { uniforms + 4 * i, 4, 0, 0 };
+        usrc_t s = { &ub, NULL };
+        accepted[i] = metropolis_link(staples + 4 * i, beta, eps, &s, links_in + 4 * i, links + 4 * i);
+    }
+    return 0;
+}
+
+int64_t or_metro_subsweep_philox(const olat_t *lat, double *U, int mu, int parity, double beta, double eps,
+                                 uint64_t seed, uint32_t sweep)
+{
+    int64_t acc = 0;
+    for (int64_t n = 0; n < lat->vol; n++) {
+        if (site_parity(lat, n) != parity) continue;
+        olink_rng_t rng;
+        lrng_init(&rng, seed, (uint32_t)(n / 2), (uint32_t)(mu | (parity << 2) | (OR_KIND_METRO << 3)), sweep);
+        usrc_t s = { NULL, &rng };
+        double A[4], Un[4];
+        or_staple(lat, U, n, mu, A);
+        acc += metropolis_link(A, beta, eps, &s, CLINK(U, n, mu), Un);
+        memcpy(LINK(U, n, mu), Un, sizeof(Un));
+    }
+    return acc;
+}
+
+double or_metropolis_update(const olat_t *lat, double *U, int nsweeps, double beta, double eps, uint64_t seed,
+                            uint32_t sweep0)
+{
+    int64_t acc = 0;
+    for (int s = 0; s < nsweeps; s++)
+        for (int mu = 0; mu < 4; mu++)
+            for (int p = 0; p < 2; p++) acc += or_metro_subsweep_philox(lat, U, mu, p, beta, eps, seed, sweep0 + s);
+    or_project_field(lat, U);
+    return nsweeps > 0 ? (double)acc / (double)(lat->vol * 4 * nsweeps) : 0.0;
 }
 
 /* Haar-uniform start (Shoemake, "Uniform random rotations", Graphics Gems

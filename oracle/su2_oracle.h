@@ -94,6 +94,24 @@ void or_or_link(const double A[4], const double U[4], double Unew[4]);
  * sweeps in the same order; with n_or == 0 a projection pass follows. */
 double or_engine_sweep(const olat_t *lat, double *U, double beta, int n_or, uint64_t seed,
                        uint32_t sweep);
+/* Metropolis with one hit (metro.c:65-109, proposal expx of exp_fct.c:137-151):
+ * X_i = 1 - 2 r_i (3 uniforms), U' = expx(eps, X) U = [((1-z) c, c eps X)] U with
+ * z = eps^2 |X|^2 / 4, c = 1/(1+z); accepted iff r_4 <= exp(beta (Re tr U'A -
+ * Re tr UA) / 2).  Writes the new (or old) link, returns 1 if accepted. */
+int or_metropolis_link(const double A[4], double beta, double eps, or_ubuf_t *ub, const double U[4],
+                       double Unew[4]);
+/* Batch form for the per-link harness: 4 uniforms per link. */
+int or_metropolis_batch(int64_t n, const double *staples, const double *links_in, double beta, double eps,
+                        const double *uniforms, double *links, int *accepted);
+/* Checkerboard Metropolis sub-sweep driven by the engine's Philox streams
+ * (kind 2); returns the number of accepted links. */
+int64_t or_metro_subsweep_philox(const olat_t *lat, double *U, int mu, int parity, double beta, double eps,
+                                 uint64_t seed, uint32_t sweep);
+/* Engine Metropolis update: nsweeps x (mu = 0..3, p = 0,1) sub-sweeps from
+ * sweep index sweep0, then one projection of the field (update.c:166);
+ * returns the mean acceptance. */
+double or_metropolis_update(const olat_t *lat, double *U, int nsweeps, double beta, double eps, uint64_t seed,
+                            uint32_t sweep0);
 /* Engine hot start: Haar-uniform links from Philox (Shoemake's method). */
 void or_hot_start(const olat_t *lat, double *U, uint64_t seed);
 void or_cold_start(const olat_t *lat, double *U);

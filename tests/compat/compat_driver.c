@@ -128,10 +128,17 @@ int main(int argc, char **argv)
     project_gfield_to_sun(pu);
     write_field(out, "project.bin");
     fprintf(s, "plaq_after %.17g\n", plaquette());
+
     long long up = 0, down = 0;
     lgt_compat_transfers(&up, &down);
     fprintf(s, "uploads %lld\ndownloads %lld\n", up, down);
     fclose(s);
+
+    /* Metropolis (SIM_TYPE=0) with the program's proposal width */
+    runParams.eps = 0.4;
+    readConfig(argv[5]);
+    gaugefieldUpdate(2, 2, 0, 0);
+    write_field(out, "metro.bin");
     releaseGaugeField();
     lgt_compat_release();
     return 0;

@@ -79,6 +79,11 @@ class Oracle:
                 "or_qtop_clover": (C.c_double, [P, _dp, C.c_int, _dp, _dp]),
                 "or_wloop_objective": (C.c_double, [P, _dp, C.c_int, C.c_int]),
                 "or_ape_vjp": (C.c_double, [P, _dp, C.c_int, C.c_double, C.c_int, C.c_int, _dp]),
+                "or_metropolis_batch": (C.c_int, [C.c_int64, _dp, _dp, C.c_double, C.c_double, _dp, _dp, _ip]),
+                "or_metro_subsweep_philox": (C.c_int64, [P, _dp, C.c_int, C.c_int, C.c_double, C.c_double,
+                                                          C.c_uint64, C.c_uint32]),
+                "or_metropolis_update": (C.c_double, [P, _dp, C.c_int, C.c_double, C.c_double, C.c_uint64,
+                                                       C.c_uint32]),
             }
             for name, (res, args) in sig.items():
                 f = getattr(lib, name)
@@ -139,6 +144,14 @@ class Oracle:
     def engine_sweep(self, U, beta, n_or, seed, sweep):
         return self.lib().or_engine_sweep(C.byref(self.lat), _ptr(U), beta, n_or, C.c_uint64(seed), sweep)
 
+    def metro_subsweep(self, U, mu, parity, beta, eps, seed, sweep):
+        return self.lib().or_metro_subsweep_philox(C.byref(self.lat), _ptr(U), mu, parity, beta, eps,
+                                                   C.c_uint64(seed), sweep)
+
+    def metropolis_update(self, U, nsweeps, beta, eps, seed, sweep0=0):
+        return self.lib().or_metropolis_update(C.byref(self.lat), _ptr(U), nsweeps, beta, eps, C.c_uint64(seed),
+                                               sweep0)
+
     def project(self, U):
         self.lib().or_project_field(C.byref(self.lat), _ptr(U))
 
@@ -189,6 +202,19 @@ def philox(ctr, key):
     return tuple(o)
 
 
+def metropolis_batch(staples, links, beta, eps, uniforms):
+    """Oracle Metropolis hit per link (or_metropolis_batch): 4 uniforms per link."""
+    staples = np.ascontiguousarray(staples, dtype=np.float64)
+    links = np.ascontiguousarray(links, dtype=np.float64)
+    uniforms = np.ascontiguousarray(uniforms, dtype=np.float64)
+    n = len(staples)
+    out = np.zeros((n, 4))
+    acc = np.zeros(n, dtype=np.int32)
+    Oracle.lib().or_metropolis_batch(n, _ptr(staples), _ptr(links), beta, eps, _ptr(uniforms), _ptr(out),
+                                     _ptr(acc, C.c_int))
+    return out, acc
+
+
 def heatbath_batch(staples, beta, uniforms):
     staples = np.ascontiguousarray(staples, dtype=np.float64)
     uniforms = np.ascontiguousarray(uniforms, dtype=np.float64)
@@ -233,6 +259,7 @@ class RefLib:
                 "ref_rng_draw": (None, [_dp, C.c_int]),
                 "ref_staple": (None, [C.c_long, C.c_int, _dp]),
                 "ref_heatbath_link": (C.c_double, [C.c_long, C.c_int]),
+                "ref_metropolis_link": (C.c_double, [C.c_long, C.c_int, C.c_double]),
                 "ref_update": (None, [C.c_int]),
                 "ref_project": (None, []),
                 "ref_plaquette": (C.c_double, []),

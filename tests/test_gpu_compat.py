@@ -73,6 +73,13 @@ def test_reference_entry_points_through_compat(tmp_path, prec, sync):
     else:
         G = load("update.bin")
         np.testing.assert_allclose(np.linalg.norm(G, axis=-1), 1.0, atol=3e-7)
+    # Metropolis after the 2 heatbath sweeps: sweep indices 2 and 3
+    V = U.copy()
+    o.metropolis_update(V, 2, 3.09, 0.4, SEED, 2)
+    if prec == "fp64":
+        assert np.max(np.abs(load("metro.bin") - V)) <= 1e-12
+    else:
+        np.testing.assert_allclose(np.linalg.norm(load("metro.bin"), axis=-1), 1.0, atol=3e-7)
     if sync == "lazy":
         # two uploads (after each readConfig), a download per writeConfig
         assert (s["uploads"], s["downloads"]) == (2, 5)

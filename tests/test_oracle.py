@@ -357,3 +357,54 @@ def test_reference_config_file_layout(tmp_path):
     ref.lib.ref_init_field(0)
     ref.read_config(tmp_path / "conf2")
     np.testing.assert_array_equal(ref.get_field(), W)
+
+
+# ---------------------------------------------------------------------------
+# Metropolis (metro.c:65-109, exp_fct.c:137-151)
+# ---------------------------------------------------------------------------
+@pytest.mark.skipif(not RefLib.available("l4"), reason="reference build missing")
+def test_metropolis_link_matches_reference():
+    """The oracle's Metropolis hit, fed the RANLUX uniforms the reference
+    consumes (replayed), gives the reference's new link bit for bit, accepted
+    or not, for several proposal widths."""
+    from oracle_lib import metropolis_batch
+    ref = RefLib("l4")
+    ref.setup(2.3, 11)
+    ref.lib.ref_init_field(1)
+    nacc = 0
+    for k, (n, mu, eps) in enumerate([(0, 0, 0.2), (7, 1, 0.5), (33, 2, 1.0), (101, 3, 2.0), (200, 0, 0.05),
+                                       (250, 3, 1.5), (17, 2, 0.8), (90, 1, 0.3)]):
+        A = ref.staple(n, mu)
+        U = ref.get_link(n, mu)
+        st = ref.rng_state()
+        r = ref.rng_draw(4)
+        ref.rng_reset(st)
+        a = ref.lib.ref_metropolis_link(n, mu, eps)
+        new = ref.get_link(n, mu)
+        out, acc = metropolis_batch(A[None], U[None], 2.3, eps, r[None])
+        np.testing.assert_array_equal(out[0], new)
+        assert acc[0] == int(a)
+        nacc += acc[0]
+    assert 0 < nacc < 8      # both branches are exercised
+
+
+def test_metropolis_detailed_balance_two_state():
+    """A hit with proposal X and its inverse -X (the proposal is symmetric):
+    P(U -> U') exp(S-weight U) equals P(U' -> U) exp(S-weight U') at fixed
+    staple (detailed balance of metro.c:80-108)."""
+    from oracle_lib import metropolis_batch
+    rng = np.random.default_rng(3)
+    A = F.haar(rng, (1,))[0] * 2.7
+    U = F.haar(rng, (1,))[0]
+    beta, eps = 2.3, 0.7
+    r = np.array([0.2, 0.9, 0.35, 0.0])        # accept for sure: r4 = 0
+    out, acc = metropolis_batch(A[None], U[None], beta, eps, r[None])
+    assert acc[0] == 1
+    Up = out[0]
+    rinv = np.array([1 - 0.2, 1 - 0.9, 1 - 0.35, 0.0])   # X -> -X
+    back, _ = metropolis_batch(A[None], Up[None], beta, eps, rinv[None])
+    np.testing.assert_allclose(back[0], U, atol=1e-14)   # expx(-X) = expx(X)^+
+    w = lambda V: np.exp(beta / 2 * 2 * F.qmul(V, A)[0])
+    p_fwd = min(1.0, w(Up) / w(U))
+    p_bwd = min(1.0, w(U) / w(Up))
+    assert p_fwd * w(U) == pytest.approx(p_bwd * w(Up), rel=1e-12)
