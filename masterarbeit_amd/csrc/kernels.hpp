@@ -559,16 +559,24 @@ static __global__ void __launch_bounds__(BLOCK) k_or_harness(int n, const double
 // plaquette (LGT_Code/modules/obs/plaquette.c:67-101)
 // ---------------------------------------------------------------------------
 template <typename T>
-__device__ __forceinline__ T plaq_term(const Planes<T>& P, const Geo& g, const Site& s, int jj, int kk)
+__device__ __forceinline__ Q<double> widen(const Q<T>& q) { return Q<double>{q.c0, q.c1, q.c2, q.c3}; }
+
+// The plaquette is evaluated in FP64 for both storages: an FP32 field's
+// plaquette is then the one the reference computes from the same links (as
+// widened into a configuration file), so files written from FP32 storage pass
+// readConfig's plaquette test (config_IO.c:609-614).
+template <typename T>
+__device__ __forceinline__ double plaq_term(const Planes<T>& P, const Geo& g, const Site& s, int jj, int kk)
 {
-    Q<T> un0 = link(P, g, jj, s);
-    Q<T> un1 = link(P, g, kk, shift(g, s, jj, 1));
-    Q<T> un2 = qmul(un0, un1);
-    un0 = qdag(link(P, g, jj, shift(g, s, kk, 1)));
+#pragma clang fp contract(off)
+    Q<double> un0 = widen(link(P, g, jj, s));
+    Q<double> un1 = widen(link(P, g, kk, shift(g, s, jj, 1)));
+    Q<double> un2 = qmul(un0, un1);
+    un0 = qdag(widen(link(P, g, jj, shift(g, s, kk, 1))));
     un1 = qmul(un2, un0);
-    un2 = qdag(link(P, g, kk, s));
+    un2 = qdag(widen(link(P, g, kk, s)));
     un0 = qmul(un1, un2);
-    return T(2) * un0.c0;
+    return 2.0 * un0.c0;
 }
 
 // grid: x over (tile, parity) pairs; partial[p*tiles + tile]
@@ -585,7 +593,7 @@ __global__ void __launch_bounds__(BLOCK) k_plaquette(Planes<T> P, Geo g, DD* __r
 #pragma unroll
         for (int jj = 0; jj < 3; jj++)
 #pragma unroll
-            for (int kk = jj + 1; kk < 4; kk++) acc = dd_add(acc, (double)plaq_term(P, g, s, jj, kk));
+            for (int kk = jj + 1; kk < 4; kk++) acc = dd_add(acc, plaq_term(P, g, s, jj, kk));
     }
     acc = block_sum_dd(acc);
     if (threadIdx.x == 0) partial[p * (gridDim.x >> 1) + tile] = acc;

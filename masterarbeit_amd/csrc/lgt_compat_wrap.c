@@ -9,6 +9,10 @@
  * LGT_Code/modules/io/config_IO.c:40-117 writeConfig, :348-535
  * readConfigForThermalisation, :537-621 readConfig).  The __real_ references
  * are weak: a function the program does not wrap is never reached here.
+ *
+ * pu is marked changed before a function that writes it runs: readConfig
+ * ends with its plaquette test, a call of the engine, which must then see
+ * the new links (its upload clears the mark, so nothing is sent twice).
  */
 #include "../../include/lgt_compat.h"
 
@@ -22,29 +26,29 @@ extern void __real_copyGaugeField(su2mat *u1[][4], su2mat *u2[][4]) __attribute_
 void __wrap_initGaugeField(int start)
 {
     lgt_compat_lazy();
-    __real_initGaugeField(start);
     lgt_compat_host_modified();
+    __real_initGaugeField(start);
 }
 
 void __wrap_releaseGaugeField(void)
 {
     lgt_compat_lazy();
-    __real_releaseGaugeField();
     lgt_compat_host_modified();
+    __real_releaseGaugeField();
 }
 
 void __wrap_readConfig(char *name)
 {
     lgt_compat_lazy();
-    __real_readConfig(name);
     lgt_compat_host_modified();
+    __real_readConfig(name);
 }
 
 void __wrap_readConfigForThermalisation(char *name)
 {
     lgt_compat_lazy();
-    __real_readConfigForThermalisation(name);
     lgt_compat_host_modified();
+    __real_readConfigForThermalisation(name);
 }
 
 void __wrap_writeConfig(char *name)

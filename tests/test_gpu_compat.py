@@ -88,3 +88,45 @@ def test_reference_entry_points_through_compat(tmp_path, prec, sync):
     log = open(tmp_path / "log.txt").read()
     assert "Update 1 done:\n(algorithm 1; sweep 0; acc=" in log and "update took" in log
     assert "calculating topological charge" in log
+
+
+QCD = os.path.join(ROOT, "oracle", "_ref", "qcd_l8", "qcd")
+
+
+@pytest.mark.skipif(not os.path.exists(QCD), reason="reference program not built (oracle/Makefile ref)")
+@pytest.mark.parametrize("prec", ["fp64", "fp32"])
+def test_reference_qcd_program_on_the_engine(tmp_path, prec):
+    """The reference's own main/qcd.c (with its input, I/O and admin modules,
+    compiled unchanged at 8^4) linked against liblgt_compat.so and the
+    lifecycle interceptors: it reads its input file, thermalises from a hot
+    start on the GPU, logs its plaquettes and writes configurations that the
+    engine reads back with the reference's plaquette test."""
+    import config_format as CF
+    import masterarbeit_amd as lgt
+    out = tmp_path / "out"
+    out.mkdir()
+    (tmp_path / "run.in").write_text(
+        f"outFileId     1\noutputDir     {out}\nstart         1\nbeta          3.09\nepsilon       0.15\n"
+        "nconf         12\nntherm        4\nndecorr       1\nwriteConfFreq 4\nseed          1337\n")
+    env = dict(os.environ, LGT_N_OR="1", LGT_PRECISION=prec)
+    r = subprocess.run([QCD, "-i", str(tmp_path / "run.in")], cwd=tmp_path, env=env, capture_output=True, text=True,
+                       timeout=120)
+    assert r.returncode == 0, r.stdout[-2000:] + r.stderr[-2000:]
+    logs = list(out.glob("*.log"))
+    assert len(logs) == 1
+    log = logs[0].read_text()
+    assert log.count("(algorithm 1; sweep 0; acc=") == 12
+    plaq = [float(l.split("\t")[1]) for l in log.splitlines() if l.startswith("plaq\t")]
+    assert len(plaq) == 12 and plaq[0] < plaq[-1] and 0.65 < plaq[-1] < 0.8
+    confs = sorted(out.glob("*_n*"))
+    assert [c.name.rsplit("_", 1)[1] for c in confs] == ["n1", "n2"]
+    lgt.load_library()
+    for c, step in zip(confs, (7, 11)):
+        dim, sun, L, p, U = CF.read(c)
+        assert (dim, sun, L) == (4, 2, (8, 8, 8, 8))
+        assert p == pytest.approx(plaq[step], abs=1e-8)
+        # the FP32 engine evaluates the plaquette in FP64 too: the files of
+        # either storage pass the FP64 engine's (the reference's) test
+        with lgt.Engine(L, 0, 1) as e:
+            p0, p1 = e.read_config(c)
+        assert p0 == p
