@@ -130,3 +130,34 @@ def test_reference_qcd_program_on_the_engine(tmp_path, prec):
         with lgt.Engine(L, 0, 1) as e:
             p0, p1 = e.read_config(c)
         assert p0 == p
+
+
+QTOP = os.path.join(ROOT, "oracle", "_ref", "qtop_l8", "qtop")
+
+
+@pytest.mark.skipif(not os.path.exists(QTOP), reason="reference program not built (oracle/Makefile ref)")
+def test_reference_qtop_program_on_the_engine(tmp_path):
+    """The reference's main_qtop/qtop.c compiled unchanged against the drop-in
+    library: it reads a configuration, alternates 4D APE steps (sp = 0.1) with
+    the improved charge 21 times and logs QTOP lines; the charges equal the
+    FP64 oracle's on the same configuration."""
+    import masterarbeit_amd as lgt
+    lgt.load_library()
+    L = (8, 8, 8, 8)
+    with lgt.Engine(L, 0, 9) as e:
+        e.init_field(1)
+        e.update(30, 1, 2.3)
+        U = e.download()
+        e.write_config(tmp_path / "8x8x8x8_SU2_b2.300000_id1_n1")
+    r = subprocess.run([QTOP, str(tmp_path)], cwd=tmp_path, capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stdout[-2000:] + r.stderr[-2000:]
+    meas = (tmp_path / "8x8x8x8_SU2_b2.300000_id1.meas").read_text()
+    got = [(int(l.split()[2]), float(l.split()[4])) for l in meas.splitlines() if l.startswith("QTOP ")]
+    assert [n for n, _ in got] == list(range(21))
+    o = Oracle(L)
+    V = U.copy()
+    for n, q in got:
+        qo, qs = o.qtop(V)
+        assert q == pytest.approx(qo, abs=max(1e-8 * abs(qo), 1e-13 * qs))
+        o.ape(V, 1, 1, 0.1)
+    assert abs(got[-1][1]) > 1e-3     # smearing leaves a non-trivial charge density
