@@ -161,3 +161,52 @@ def test_reference_qtop_program_on_the_engine(tmp_path):
         assert q == pytest.approx(qo, abs=max(1e-8 * abs(qo), 1e-13 * qs))
         o.ape(V, 1, 1, 0.1)
     assert abs(got[-1][1]) > 1e-3     # smearing leaves a non-trivial charge density
+
+
+MEAS = os.path.join(ROOT, "oracle", "_ref", "meas_l8", "meas")
+
+
+@pytest.mark.skipif(not os.path.exists(MEAS), reason="reference program not built (oracle/Makefile ref)")
+def test_reference_meas_program_on_the_engine(tmp_path):
+    """The reference's main_meas/meas.c compiled unchanged against the drop-in
+    library (2-pt functions off): for two configurations it logs the charge
+    after 4D APE and the Wilson loops after spatial APE, equal to the FP64
+    oracle's on the same configurations."""
+    import masterarbeit_amd as lgt
+    lgt.load_library()
+    L = (8, 8, 8, 8)
+    base = tmp_path / "8x8x8x8_SU2_b2.300000_id1"
+    fields = []
+    with lgt.Engine(L, 0, 13) as e:
+        e.init_field(1)
+        for n in (1, 2):
+            e.update(15, 1, 2.3)
+            fields.append(e.download())
+            e.write_config(f"{base}_n{n}")
+    (tmp_path / "meas.in").write_text(
+        f"outFileId      1\nodir           {tmp_path}\nbeta           2.3\nseed           1\nsconf          1\n"
+        "fconf          2\neconf          1\nmtopcharge     1\nsmear_tc       10 0.1\nmwil           1 0\n"
+        "tLoopExtent    1 4 1\ntSmear         0 0.0\nrLoopExtent    1 3 1\nrSmear         5 0.25\n"
+        "mcorrs         0\nkappa          0.12\nsource         0 0 0 0\nstype          0\nnmax           100\n"
+        "prec           1e-8\nn2pt           0\n")
+    r = subprocess.run([MEAS, "-i", str(tmp_path / "meas.in")], cwd=tmp_path, capture_output=True, text=True,
+                       timeout=120)
+    assert r.returncode == 0, r.stdout[-2000:] + r.stderr[-2000:]
+    meas = (tmp_path / "8x8x8x8_SU2_b2.300000_id1.meas").read_text().splitlines()
+    qs = [float(l.split()[1]) for l in meas if l.startswith("QTOP ")]
+    ws = [(int(l.split()[1]), int(l.split()[2]), float(l.split()[3])) for l in meas if l.startswith("WIL ")]
+    assert len(qs) == 2 and len(ws) == 2 * 12
+    o = Oracle(L)
+    R = [r_ for r_ in (1, 2, 3) for _ in (1, 2, 3, 4)]
+    T = [t_ for _ in (1, 2, 3) for t_ in (1, 2, 3, 4)]
+    for k, U in enumerate(fields):
+        V = U.copy()
+        o.ape(V, 1, 10, 0.1)
+        q, qsum = o.qtop(V)
+        assert qs[k] == pytest.approx(q, abs=max(1e-8 * abs(q), 1e-13 * qsum))
+        V = U.copy()
+        o.ape(V, 0, 5, 0.25)
+        w = o.wilson(V, R, T)
+        got = ws[12 * k:12 * (k + 1)]
+        assert [(a, b) for a, b, _ in got] == list(zip(R, T))
+        np.testing.assert_allclose([c for _, _, c in got], w, rtol=1e-8)
