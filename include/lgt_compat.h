@@ -20,7 +20,8 @@
  *  - lazy: the device field is authoritative; entry points upload pu only when
  *    the program changed it and download nothing.  The program is linked with
  *    -Wl,--wrap=initGaugeField,--wrap=readConfig,--wrap=readConfigForThermalisation,
- *    --wrap=writeConfig,--wrap=copyGaugeField,--wrap=releaseGaugeField so
+ *    --wrap=writeConfig,--wrap=copyGaugeField,--wrap=releaseGaugeField (and
+ *    --wrap=measure2ptFunctions where the fermion modules are linked) so
  *    that its own field-lifecycle functions (modules.h:119-142) pass through
  *    the __wrap_ functions below, which download before pu is read and mark
  *    pu changed after it is written.  The first intercepted call selects lazy
@@ -121,6 +122,7 @@ void __wrap_readConfig(char *name);
 void __wrap_readConfigForThermalisation(char *name);
 void __wrap_writeConfig(char *name);
 void __wrap_copyGaugeField(su2mat *u1[][4], su2mat *u2[][4]);
+void __wrap_measure2ptFunctions(int *source, int stype, int n2pt, int *g1, int *g2);
 
 #ifdef __cplusplus
 }

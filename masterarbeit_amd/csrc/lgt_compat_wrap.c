@@ -7,7 +7,8 @@
  * program's own definition (LGT_Code/modules/admin/init.c:831-869 initGaugeField,
  * :897-905 copyGaugeField, :922-925 releaseGaugeField;
  * LGT_Code/modules/io/config_IO.c:40-117 writeConfig, :348-535
- * readConfigForThermalisation, :537-621 readConfig).  The __real_ references
+ * readConfigForThermalisation, :537-621 readConfig) and of the host-side
+ * readers of pu outside the engine's path (obs/2pt.c measure2ptFunctions).  The __real_ references
  * are weak: a function the program does not wrap is never reached here.
  *
  * pu is marked changed before a function that writes it runs: readConfig
@@ -22,6 +23,7 @@ extern void __real_readConfig(char *) __attribute__((weak));
 extern void __real_readConfigForThermalisation(char *) __attribute__((weak));
 extern void __real_writeConfig(char *) __attribute__((weak));
 extern void __real_copyGaugeField(su2mat *u1[][4], su2mat *u2[][4]) __attribute__((weak));
+extern void __real_measure2ptFunctions(int *source, int stype, int n2pt, int *g1, int *g2) __attribute__((weak));
 
 void __wrap_initGaugeField(int start)
 {
@@ -64,4 +66,12 @@ void __wrap_copyGaugeField(su2mat *u1[][4], su2mat *u2[][4])
     if (u1 == pu) lgt_compat_sync_host();
     __real_copyGaugeField(u1, u2);
     if (u2 == pu) lgt_compat_host_modified();
+}
+
+/* the fermionic 2-pt functions (obs/2pt.c, modules.h:49) read pu on the host */
+void __wrap_measure2ptFunctions(int *source, int stype, int n2pt, int *g1, int *g2)
+{
+    lgt_compat_lazy();
+    lgt_compat_sync_host();
+    __real_measure2ptFunctions(source, stype, n2pt, g1, g2);
 }

@@ -49,7 +49,7 @@ def test_compat_exports_reference_names():
     assert {"pu", "runParams", "latParams", "measParams", "logging", "error", "getTime", "ranlxd"} <= undefined
     # the lifecycle interceptors live in the static archive linked into the program
     wraps = {n for n in declared_names if n.startswith("__wrap_")}
-    assert len(wraps) == 6
+    assert len(wraps) == 7
     archive = os.path.join(ROOT, "masterarbeit_amd", "liblgt_compat_wrap.a")
     nm = subprocess.run(["nm", archive], capture_output=True, text=True, check=True).stdout
     assert wraps <= {l.split()[-1] for l in nm.splitlines() if " T " in l}
