@@ -23,6 +23,14 @@
 namespace lgt {
 
 
+// keeps q's value in registers at this point of the program: what computes it
+// is not moved past the following statements
+template <typename T>
+__device__ __forceinline__ void pin(Q<T>& q)
+{
+    asm volatile("" : "+v"(q.c0), "+v"(q.c1), "+v"(q.c2), "+v"(q.c3));
+}
+
 // H = (G - <G,U> U) / |M| on the spatial links
 template <typename T>
 __global__ void __launch_bounds__(BLOCK) k_vjp_tangent(Planes<T> U, Planes<T> G, NormPlanes<T> Mn, Planes<T> H, Geo g)
@@ -42,8 +50,13 @@ __global__ void __launch_bounds__(BLOCK) k_vjp_tangent(Planes<T> U, Planes<T> G,
 // its output (H above).  Writes G = dL/dZ for the spatial links, or, when
 // Mprev is given (|M| of the step before), directly the tangent cotangent of
 // that earlier step: (G - <G,Z> Z)/Mprev  (Z is that step's output).
-// One thread handles the three spatial links of its site, so the gathers the
-// directions share are loaded once (51 instead of 81).
+// One thread handles the three spatial links of its site and loads each of
+// the 48 (Z, H) links they need once, in batches: the six at y; for every
+// pair {a,b} the four at y+a and y+b, which close the forward staples of both
+// G_a and G_b; for every b the ten around y-b, which close the backward
+// staples of the two G_a with a != b.  A batch's loads are issued together
+// before its products (its latency is exposed once) and only one batch is
+// live at a time (FP64: 166 VGPRs, three waves per SIMD).
 // grid: x over (tile, parity) pairs
 template <typename T>
 __global__ void __launch_bounds__(BLOCK) k_vjp_step(Planes<T> Z, Planes<T> Hin, NormPlanes<T> Mprev, int has_prev,
@@ -53,58 +66,74 @@ __global__ void __launch_bounds__(BLOCK) k_vjp_step(Planes<T> Z, Planes<T> Hin, 
     int p;
     uint32_t i = tile_base_pair(g, p) + threadIdx.x;
     if (i >= g.vh) return;
-    Site y = cb_site(g, i, p);
-    Site yp[4], ym[4];
-#pragma unroll
-    for (int b = 1; b < 4; b++) {
-        yp[b] = shift(g, y, b, 1);
-        ym[b] = shift(g, y, b, -1);
-    }
-    Q<T> res[4];
+    const Site y = cb_site(g, i, p);
+    Q<T> Zy[4], Hy[4], acc[4];
 #pragma unroll
     for (int a = 1; a < 4; a++) {
-        Q<T> acc = qzero<T>();
-        // the 24 gathers of both b != a are issued before any product: the
-        // latency of one batch is exposed once instead of per term
-        const int b0 = (a == 1) ? 2 : 1, b1 = (a == 3) ? 2 : 3;
-        Q<T> L[2][12];
+        Zy[a] = link(Z, g, a, y);
+        Hy[a] = link(Hin, g, a, y);
+        acc[a] = qzero<T>();
+    }
+    // batches 1-3: for each pair {a,b} the forward staples through y+a, y+b
 #pragma unroll
-        for (int j = 0; j < 2; j++) {
-            const int b = j ? b1 : b0;
-            Site ymba = shift(g, ym[b], a, 1);
-            L[j][0] = link(Z, g, b, y);       L[j][1] = link(Hin, g, b, y);
-            L[j][2] = link(Z, g, a, yp[b]);   L[j][3] = link(Hin, g, a, yp[b]);
-            L[j][4] = link(Z, g, b, yp[a]);   L[j][5] = link(Hin, g, b, yp[a]);
-            L[j][6] = link(Z, g, b, ym[b]);   L[j][7] = link(Hin, g, b, ym[b]);
-            L[j][8] = link(Z, g, a, ym[b]);   L[j][9] = link(Hin, g, a, ym[b]);
-            L[j][10] = link(Z, g, b, ymba);   L[j][11] = link(Hin, g, b, ymba);
+    for (int a = 1; a < 4; a++)
+#pragma unroll
+        for (int b = a + 1; b < 4; b++) {
+#pragma unroll
+            for (int c = 1; c < 4; c++) pin(acc[c]);
+            asm volatile("" ::: "memory");
+            const Site ya = shift(g, y, a, 1), yb = shift(g, y, b, 1);
+            const Q<T> Zab = link(Z, g, a, yb), Hab = link(Hin, g, a, yb);   // Z_a(y+b), H_a(y+b)
+            const Q<T> Zba = link(Z, g, b, ya), Hba = link(Hin, g, b, ya);   // Z_b(y+a), H_b(y+a)
+            asm volatile("" ::: "memory");
+            // t1 + t5 + t4 of G_a = (H_b(y) Z_a(y+b) + Z_b(y) H_a(y+b)) Z_b(y+a)^+ + Z_b(y) Z_a(y+b) H_b(y+a)^+
+            Q<T> up = qadd(qmul(Hy[b], Zab), qmul(Zy[b], Hab));
+            acc[a] = qadd(acc[a], qadd(qmul_dag(up, Zba), qmul_dag(qmul(Zy[b], Zab), Hba)));
+            // the same for G_b, a and b exchanged
+            up = qadd(qmul(Hy[a], Zba), qmul(Zy[a], Hba));
+            acc[b] = qadd(acc[b], qadd(qmul_dag(up, Zab), qmul_dag(qmul(Zy[a], Zba), Hab)));
+        }
+    // batches 4-6: for each b the backward staples through y-b
+#pragma unroll
+    for (int b = 1; b < 4; b++) {
+        // the previous batch's products are complete before this batch's loads
+        // are issued (its links are dead by then: one batch live at a time)
+#pragma unroll
+        for (int a = 1; a < 4; a++) pin(acc[a]);
+        asm volatile("" ::: "memory");
+        const Site ym = shift(g, y, b, -1);
+        const Q<T> Zb = link(Z, g, b, ym), Hb = link(Hin, g, b, ym);   // Z_b(y-b), H_b(y-b)
+        Q<T> C[4][2], D[4][2];   // C[a] = (Z_a(y-b), H_a(y-b)), D[a] = (Z_b(y-b+a), H_b(y-b+a))
+#pragma unroll
+        for (int a = 1; a < 4; a++) {
+            if (a == b) continue;
+            const Site yma = shift(g, ym, a, 1);
+            C[a][0] = link(Z, g, a, ym);
+            C[a][1] = link(Hin, g, a, ym);
+            D[a][0] = link(Z, g, b, yma);
+            D[a][1] = link(Hin, g, b, yma);
         }
         asm volatile("" ::: "memory");
+        const Q<T> Zbd = qdag(Zb), Hbd = qdag(Hb);
 #pragma unroll
-        for (int j = 0; j < 2; j++) {
-            // forward staple: t1 + t5 = (H_b(y) Z_a(y+b) + Z_b(y) H_a(y+b)) Z_b(y+a)^+,
-            //                 t4 = Z_b(y) Z_a(y+b) H_b(y+a)^+   (5 products, not 6)
-            Q<T> up = qadd(qmul(L[j][1], L[j][2]), qmul(L[j][0], L[j][3]));
-            Q<T> zz = qmul(L[j][0], L[j][2]);
-            acc = qadd(acc, qmul_dag(up, L[j][4]));
-            acc = qadd(acc, qmul_dag(zz, L[j][5]));
-            // backward staple: t2 + t3 = (Z_b(y-b)^+ H_a(y-b) + H_b(y-b)^+ Z_a(y-b)) Z_b(y-b+a),
-            //                  t6 = Z_b(y-b)^+ Z_a(y-b) H_b(y-b+a)
-            Q<T> lo = qadd(qmul(qdag(L[j][6]), L[j][9]), qmul(qdag(L[j][7]), L[j][8]));
-            Q<T> zl = qmul(qdag(L[j][6]), L[j][8]);
-            acc = qadd(acc, qmul(lo, L[j][10]));
-            acc = qadd(acc, qmul(zl, L[j][11]));
+        for (int a = 1; a < 4; a++) {
+            if (a == b) continue;
+            // t2 + t3 + t6 = (Z_b(y-b)^+ H_a(y-b) + H_b(y-b)^+ Z_a(y-b)) Z_b(y-b+a) + Z_b(y-b)^+ Z_a(y-b) H_b(y-b+a)
+            const Q<T> lo = qadd(qmul(Zbd, C[a][1]), qmul(Hbd, C[a][0]));
+            const Q<T> zl = qmul(Zbd, C[a][0]);
+            acc[a] = qadd(acc[a], qadd(qmul(lo, D[a][0]), qmul(zl, D[a][1])));
         }
-        Q<T> G = qadd(link(Hin, g, a, y), qscale(acc, sp));
+    }
+#pragma unroll
+    for (int a = 1; a < 4; a++) {
+        Q<T> G = qadd(Hy[a], qscale(acc[a], sp));
         if (has_prev) {
-            Q<T> z = link(Z, g, a, y);
+            const Q<T>& z = Zy[a];
             T c = qdot(G, z), m = Mprev.p[a][p][i];
             G = Q<T>{(G.c0 - c * z.c0) / m, (G.c1 - c * z.c1) / m, (G.c2 - c * z.c2) / m, (G.c3 - c * z.c3) / m};
         }
-        res[a] = G;
+        qstore<T>(Out.p[a][p], i, G);
     }
-#pragma unroll
-    for (int a = 1; a < 4; a++) qstore<T>(Out.p[a][p], i, res[a]);
 }
 
 // ---------------------------------------------------------------------------
