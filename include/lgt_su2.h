@@ -54,11 +54,16 @@ typedef struct lgt_engine lgt_engine;
                                 else flags 1 = each XCD takes a contiguous share of the
                                 tiles, 2 = time coordinate fastest, 4 = time fastest within
                                 groups of spatial tiles (2 and 4 apply when a time slice
-                                holds whole tiles).  Results do not depend on it. */
+                                holds whole tiles), 8 = the resulting order reversed.
+                                Results do not depend on it. */
 #define LGT_OPT_PROFILE 1    /* 1: time every launch with HIP events */
 #define LGT_OPT_GROUP 2      /* spatial tiles per group of the grouped tile order: 0 (default)
                                 = tuned for the lattice; else a divisor of the tiles in one
                                 time slice.  Results do not depend on it. */
+#define LGT_OPT_ALTERNATE 3  /* 1 (default): consecutive update launches walk the tiles in
+                                opposite orders, so each starts on what the previous one
+                                touched last; 0: every launch in the same order.  Results do
+                                not depend on it. */
 
 /* kernel classes of the launch profile (lgt_profile_read) */
 #define LGT_K_HEATBATH 0

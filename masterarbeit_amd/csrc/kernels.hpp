@@ -32,7 +32,8 @@ constexpr double PI_D = 3.14159265358979323846;   // = 2 asin(1), LGT_Code/modul
 // Returns the first cb index of the block's tile.
 __device__ __forceinline__ uint32_t xcd_remap(const Geo& g, uint32_t b, uint32_t nb)
 {
-    return ((g.order & 1) && (nb & 7u) == 0) ? (b & 7u) * (nb >> 3) + (b >> 3) : b;
+    const uint32_t l = ((g.order & 1) && (nb & 7u) == 0) ? (b & 7u) * (nb >> 3) + (b >> 3) : b;
+    return (g.order & 8) ? nb - 1u - l : l;
 }
 
 __device__ __forceinline__ uint32_t tile_map(const Geo& g, uint32_t b)

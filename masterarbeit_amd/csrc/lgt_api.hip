@@ -91,6 +91,8 @@ struct lgt_engine {
     size_t staging_bytes = 0;
     int profile = 0;
     int order = -1;   // tile order of every kernel; -1: per kernel (see kgeo)
+    int alternate = 1;  // update launches alternate between forward and reversed tile order
+    uint32_t flip = 0;  // reverses the next update launch
     std::vector<ProfEvent> events;
     double prof_ms[LGT_K_COUNT] = {0};
     int64_t prof_n[LGT_K_COUNT] = {0};
@@ -238,6 +240,18 @@ Geo kgeo(const lgt_engine* e, int cls, int mu)
     Geo g = e->geo;
     int order = (e->order >= 0) ? e->order : (cls == LGT_K_WLOOP_GRAD ? 3 : (g.gs ? 5 : 1));
     g.order = tile_order_ok(g, order);
+    return g;
+}
+
+// Geometry of an update launch (heatbath, overrelaxation, Metropolis).  A
+// sweep reads the 48^4 FP32 field (340 MB) about twice per launch pair, more
+// than the 256 MB memory-side cache holds; walking the tiles of every second
+// launch backwards starts it on the links the previous launch touched last,
+// which are still in that cache.
+Geo ugeo(lgt_engine* e, int cls, int mu)
+{
+    Geo g = kgeo(e, cls, mu);
+    if (e->alternate && (e->flip ^= 1u)) g.order |= 8;
     return g;
 }
 
@@ -458,11 +472,15 @@ int lgt_set_option(lgt_engine* e, int option, int value)
     if (!e) return fail(LGT_EINVAL, "lgt_set_option: null engine");
     switch (option) {
     case LGT_OPT_TILE_ORDER:
-        if (value < -1 || value > 7) return fail(LGT_EINVAL, "lgt_set_option: tile order %d", value);
+        if (value < -1 || value > 15) return fail(LGT_EINVAL, "lgt_set_option: tile order %d", value);
         e->order = value;
         return LGT_OK;
     case LGT_OPT_GROUP:
         if (!set_group(e->geo, value)) return fail(LGT_EINVAL, "lgt_set_option: group size %d", value);
+        return LGT_OK;
+    case LGT_OPT_ALTERNATE:
+        if (value != 0 && value != 1) return fail(LGT_EINVAL, "lgt_set_option: alternate %d", value);
+        e->alternate = value;
         return LGT_OK;
     case LGT_OPT_PROFILE:
         if (!value) TRY(flush_profile(e));
@@ -677,7 +695,7 @@ int lgt_metropolis_update(lgt_engine* e, int nsweeps, double beta, double eps, d
             for (int p = 0; p < 2; p++) {
                 void* ap = acceptance ? (char*)e->partial + sizeof(double) * 2 * (size_t)nb * (mu * 2 + p) : nullptr;
                 TRY(run(e, LGT_K_METROPOLIS, "metropolis", [&] {
-                    return DISPATCH(e, metropolis(e->stream, kgeo(e, LGT_K_METROPOLIS, mu), e->cur, mu, p, beta, eps,
+                    return DISPATCH(e, metropolis(e->stream, ugeo(e, LGT_K_METROPOLIS, mu), e->cur, mu, p, beta, eps,
                                                   e->seed, e->sweep, ap));
                 }));
             }
@@ -704,7 +722,7 @@ int lgt_heatbath_subsweep(lgt_engine* e, int mu, int parity, double beta, double
         ap = e->partial;
     }
     TRY(run(e, LGT_K_HEATBATH, "heatbath", [&] {
-        return DISPATCH(e, heatbath(e->stream, kgeo(e, LGT_K_HEATBATH, mu), e->cur, mu, parity, beta, e->seed, e->sweep, ap));
+        return DISPATCH(e, heatbath(e->stream, ugeo(e, LGT_K_HEATBATH, mu), e->cur, mu, parity, beta, e->seed, e->sweep, ap));
     }));
     if (acc_sum) TRY(reduce_to_host(e, 1, (int)nbx(e), acc_sum));
     return LGT_OK;
@@ -715,7 +733,7 @@ int lgt_overrelax_subsweep(lgt_engine* e, int mu, int parity)
     if (!e || mu < 0 || mu > 3 || parity < 0 || parity > 1) return fail(LGT_EINVAL, "lgt_overrelax_subsweep: mu/parity");
     TRY(set_dev(e));
     return run(e, LGT_K_OVERRELAX, "overrelax",
-               [&] { return DISPATCH(e, overrelax(e->stream, kgeo(e, LGT_K_OVERRELAX, mu), e->cur, mu, parity)); });
+               [&] { return DISPATCH(e, overrelax(e->stream, ugeo(e, LGT_K_OVERRELAX, mu), e->cur, mu, parity)); });
 }
 
 int lgt_project(lgt_engine* e)
@@ -750,7 +768,7 @@ int lgt_update(lgt_engine* e, int nsweeps, int n_or, double beta, double* accept
             for (int p = 0; p < 2; p++) {
                 void* ap = acceptance ? (char*)e->partial + sizeof(double) * 2 * (size_t)nb * (mu * 2 + p) : nullptr;
                 TRY(run(e, LGT_K_HEATBATH, "heatbath", [&] {
-                    return DISPATCH(e, heatbath(e->stream, kgeo(e, LGT_K_HEATBATH, mu), e->cur, mu, p, beta, e->seed, e->sweep, ap));
+                    return DISPATCH(e, heatbath(e->stream, ugeo(e, LGT_K_HEATBATH, mu), e->cur, mu, p, beta, e->seed, e->sweep, ap));
                 }));
             }
         if (acceptance) TRY(reduce_to_host(e, 1, (int)(8 * nb), &acc_rows[s]));
@@ -758,7 +776,7 @@ int lgt_update(lgt_engine* e, int nsweeps, int n_or, double beta, double* accept
             for (int mu = 0; mu < 4; mu++)
                 for (int p = 0; p < 2; p++)
                     TRY(run(e, LGT_K_OVERRELAX, "overrelax",
-                            [&] { return DISPATCH(e, overrelax(e->stream, kgeo(e, LGT_K_OVERRELAX, mu), e->cur, mu, p)); }));
+                            [&] { return DISPATCH(e, overrelax(e->stream, ugeo(e, LGT_K_OVERRELAX, mu), e->cur, mu, p)); }));
         if (n_or == 0) TRY(run(e, LGT_K_PROJECT, "project", [&] { return DISPATCH(e, project(e->stream, e->geo, e->cur)); }));
         e->sweep++;
     }

@@ -19,7 +19,7 @@ LIB_PATH = os.path.join(HERE, "liblgt_su2.so")
 FP64, FP32 = 0, 1
 APE_SPATIAL, APE_ALL, APE_TEMPORAL = 0, 1, 2
 Q_IMPROVED, Q_CLOVER = 0, 1
-OPT_TILE_ORDER, OPT_PROFILE, OPT_GROUP = 0, 1, 2
+OPT_TILE_ORDER, OPT_PROFILE, OPT_GROUP, OPT_ALTERNATE = 0, 1, 2, 3
 (K_HEATBATH, K_OVERRELAX, K_PROJECT, K_PLAQUETTE, K_APE, K_WLOOP, K_QTOP, K_REDUCE, K_FIELD, K_APE_SAVE, K_VJP,
  K_WLOOP_GRAD, K_METROPOLIS) = range(13)
 KERNEL_CLASSES = ["heatbath", "overrelax", "project", "plaquette", "ape", "wloop", "qtop", "reduce", "field",

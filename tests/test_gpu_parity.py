@@ -291,19 +291,24 @@ def test_update_matches_oracle_chain(lgt, L, n_or):
 
 @pytest.mark.parametrize("prec", [0, 1])
 def test_results_independent_of_tile_order(lgt, prec):
-    """Every tile order (XCD-contiguous, time-fastest, both, neither) gives
-    bit-identical sweeps, smearing and observables; repeated runs too."""
+    """Every tile order (XCD-contiguous, time-fastest, both, neither, each also
+    reversed, and update launches alternating or not) gives bit-identical
+    sweeps, acceptance, smearing and observables; repeated runs too."""
     L = (8, 16, 16, 16)    # a time slice holds 8 whole tiles: every order applies
     res = []
-    for order, group in ((-1, 0), (0, 0), (1, 0), (2, 0), (3, 0), (4, 0), (5, 0), (7, 0), (5, 2), (5, 4), (5, 8),
-                         (-1, 0)):
+    for order, group, alt in ((-1, 0, 1), (0, 0, 1), (1, 0, 1), (2, 0, 1), (3, 0, 1), (4, 0, 1), (5, 0, 1),
+                              (7, 0, 1), (5, 2, 1), (5, 4, 1), (5, 8, 1), (-1, 0, 0), (8, 0, 0), (13, 0, 0),
+                              (13, 4, 1), (15, 0, 1), (-1, 0, 1)):
         with engine(lgt, L, prec, seed=7) as e:
             e.set_option(0, order)
             e.set_option(lgt.engine.OPT_GROUP, group)
+            e.set_option(lgt.engine.OPT_ALTERNATE, alt)
             e.init_field(1)
-            e.update(3, 1, 3.09)
+            a = e.update(3, 1, 3.09, acceptance=True)
+            e.update(1, 0, 3.09)
+            e.metropolis_update(1, 3.09, 0.5)
             e.ape(0, 2, 0.25)
-            res.append((e.download(), e.plaquette(), e.wilson_loops([1, 2], [2, 3]).tolist(), e.qtop(0)))
+            res.append((e.download(), a, e.plaquette(), e.wilson_loops([1, 2], [2, 3]).tolist(), e.qtop(0)))
     for r in res[1:]:
         np.testing.assert_array_equal(r[0], res[0][0])
         assert r[1:] == res[0][1:]
@@ -314,6 +319,10 @@ def test_group_size_must_divide_time_slice(lgt):
         with pytest.raises(lgt.LgtError):
             e.set_option(lgt.engine.OPT_GROUP, 3)
         e.set_option(lgt.engine.OPT_GROUP, 4)
+        for opt, bad in ((lgt.engine.OPT_ALTERNATE, 2), (lgt.engine.OPT_TILE_ORDER, 16),
+                         (lgt.engine.OPT_TILE_ORDER, -2)):
+            with pytest.raises(lgt.LgtError):
+                e.set_option(opt, bad)
 
 
 # ---------------------------------------------------------------------------
