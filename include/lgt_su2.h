@@ -60,7 +60,8 @@ typedef struct lgt_engine lgt_engine;
 #define LGT_OPT_GROUP 2      /* spatial tiles per group of the grouped tile order: 0 (default)
                                 = tuned for the lattice; else a divisor of the tiles in one
                                 time slice.  Results do not depend on it. */
-#define LGT_OPT_ALTERNATE 3  /* 1 (default): consecutive update launches walk the tiles in
+#define LGT_OPT_ALTERNATE 3  /* 1 (default): consecutive update, smearing and adjoint
+                                launches walk the tiles in
                                 opposite orders, so each starts on what the previous one
                                 touched last; 0: every launch in the same order.  Results do
                                 not depend on it. */
