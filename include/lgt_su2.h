@@ -61,10 +61,9 @@ typedef struct lgt_engine lgt_engine;
                                 = tuned for the lattice; else a divisor of the tiles in one
                                 time slice.  Results do not depend on it. */
 #define LGT_OPT_ALTERNATE 3  /* 1 (default): consecutive update, smearing and adjoint
-                                launches walk the tiles in
-                                opposite orders, so each starts on what the previous one
-                                touched last; 0: every launch in the same order.  Results do
-                                not depend on it. */
+                                launches walk the tiles in opposite orders, so each starts
+                                on what the previous one touched last; 0: every launch in
+                                the same order.  Results do not depend on it. */
 
 /* kernel classes of the launch profile (lgt_profile_read) */
 #define LGT_K_HEATBATH 0
