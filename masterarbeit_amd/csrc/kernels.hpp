@@ -70,6 +70,21 @@ __device__ __forceinline__ uint32_t tile_base_pair(const Geo& g, int& p)
     return tile_map(g, b >> 1);
 }
 
+// keeps q's value in registers at this point of the program: what computes it
+// is not moved past the following statements
+template <typename T>
+__device__ __forceinline__ void pin(Q<T>& q)
+{
+    asm volatile("" : "+v"(q.c0), "+v"(q.c1), "+v"(q.c2), "+v"(q.c3));
+}
+
+// the same for a site: the addresses derived from it are computed after this
+// point, not hoisted ahead of earlier work
+__device__ __forceinline__ void pin(Site& s)
+{
+    asm volatile("" : "+v"(s.c[0]), "+v"(s.c[1]), "+v"(s.c[2]), "+v"(s.c[3]));
+}
+
 // ---------------------------------------------------------------------------
 // reductions
 // ---------------------------------------------------------------------------
@@ -796,26 +811,41 @@ __device__ __forceinline__ Q<T> clover_average(const Planes<T>& P, const Geo& g,
     Q<T> sum, t;
     Site n = x;
     bool f;
+    // each leaf's product is complete before the next leaf's links are loaded:
+    // one leaf (at most 12 links) live at a time keeps the charge kernel at
+    // several waves per SIMD instead of one
     // +d1,+d2
     f = true;
+    pin(n);
     walk(P, g, t, n, d1, m, false, f); walk(P, g, t, n, d2, m, false, f);
     walk(P, g, t, n, d1, m, true, f);  walk(P, g, t, n, d2, m, true, f);
     sum = t;
+    pin(sum);
+    asm volatile("" ::: "memory");
     // +d1,-d2
     f = true;
+    pin(n);
     walk(P, g, t, n, d2, m, true, f);  walk(P, g, t, n, d1, m, false, f);
     walk(P, g, t, n, d2, m, false, f); walk(P, g, t, n, d1, m, true, f);
     sum = qadd(sum, t);
+    pin(sum);
+    asm volatile("" ::: "memory");
     // -d1,+d2
     f = true;
+    pin(n);
     walk(P, g, t, n, d2, m, false, f); walk(P, g, t, n, d1, m, true, f);
     walk(P, g, t, n, d2, m, true, f);  walk(P, g, t, n, d1, m, false, f);
     sum = qadd(sum, t);
+    pin(sum);
+    asm volatile("" ::: "memory");
     // -d1,-d2
     f = true;
+    pin(n);
     walk(P, g, t, n, d1, m, true, f);  walk(P, g, t, n, d2, m, true, f);
     walk(P, g, t, n, d1, m, false, f); walk(P, g, t, n, d2, m, false, f);
     sum = qadd(sum, t);
+    pin(sum);
+    asm volatile("" ::: "memory");
     return qdiv(sum, T(4));
 }
 

@@ -23,14 +23,6 @@
 namespace lgt {
 
 
-// keeps q's value in registers at this point of the program: what computes it
-// is not moved past the following statements
-template <typename T>
-__device__ __forceinline__ void pin(Q<T>& q)
-{
-    asm volatile("" : "+v"(q.c0), "+v"(q.c1), "+v"(q.c2), "+v"(q.c3));
-}
-
 // H = (G - <G,U> U) / |M| on the spatial links
 template <typename T>
 __global__ void __launch_bounds__(BLOCK) k_vjp_tangent(Planes<T> U, Planes<T> G, NormPlanes<T> Mn, Planes<T> H, Geo g)
