@@ -710,28 +710,50 @@ __global__ void __launch_bounds__(BLOCK) k_ape(Planes<T> Z, Planes<T> O, Geo g, 
 // wilson.c:21-56,268-324), fused over all T of one R:
 //   W(x) = Re tr[ L_T(x) S_R(x+T t) L_T(x+R d)^+ S_R(x)^+ ]
 //        = 2 < L_T(x) S_R(x+T t), S_R(x) L_T(x+R d) >
-// with temporal lines L_T grown one link per step and the spatial line at
-// x+T t rebuilt per step.  Results per (group, d, parity, tile) are reduced in
+// with temporal lines L_T grown one link per step and the spatial lines S_R
+// read from a line field built beforehand (k_sline), one load per step
+// instead of R.  Results per (group, d, parity, tile) are reduced in
 // double-double; every wave adds its shuffle-reduced value to its own LDS slot.
 // ---------------------------------------------------------------------------
 struct WGroup { int R, Tmax, slot_off, pad; };
 constexpr int WMAXT = 128;
 
+// Line field: S.p[d] holds S_R(y) = U_d(y) U_d(y+d) ... U_d(y+(R-1)d) for
+// d = 1..3, the product taken left to right as wilson.c:21-56 builds the
+// line.  Advances it in place from R0 (0: empty) to R1 > R0 (each site reads
+// and writes only its own line).
+// grid: x over (tile, parity) pairs, y = d - 1
 template <typename T>
-__device__ __forceinline__ Q<T> spatial_line(const Planes<T>& P, const Geo& g, Site y, int d, int R)
+__global__ void __launch_bounds__(BLOCK) k_sline(Planes<T> P, Planes<T> S, Geo g, int R0, int R1)
 {
-    Q<T> S = link(P, g, d, y);
-    for (int j = 1; j < R; j++) {
-        y = shift(g, y, d, 1);
-        S = qmul(S, link(P, g, d, y));
+    int p;
+    uint32_t i = tile_base_pair(g, p) + threadIdx.x;
+    if (i >= g.vh) return;
+    const int d = 1 + (int)blockIdx.y;
+    const Site y = cb_site(g, i, p);
+    Q<T> line;
+    int j = R0;
+    if (R0 == 0) {
+        line = link(P, g, d, y);
+        j = 1;
+    } else {
+        line = qload<T>(S.p[d][p], i);
     }
-    return S;
+    Site yj = shift(g, y, d, j % g.L[d]);
+    for (; j < R1; j++) {
+        line = qmul(line, link(P, g, d, yj));
+        yj = shift(g, yj, d, 1);
+    }
+    typename Store<T>::V v;
+    v.x = line.c0; v.y = line.c1; v.z = line.c2; v.w = line.c3;
+    S.p[d][p][i] = v;   // read back by the loops right after: a cached store
 }
 
-// grid: x over cb sites, y = parity, z = group*3 + (d-1)
+// One group (one R, all its T) per launch; S holds the lines S_R.
+// grid: x over (tile, parity) pairs, y = d - 1
 template <typename T>
-__global__ void __launch_bounds__(BLOCK) k_wloops(Planes<T> P, Geo g, const WGroup* __restrict__ groups,
-                                                  const int* __restrict__ slots, DD* __restrict__ partial,
+__global__ void __launch_bounds__(BLOCK) k_wloops(Planes<T> P, Planes<T> S, Geo g, const WGroup* __restrict__ groups,
+                                                  int group, const int* __restrict__ slots, DD* __restrict__ partial,
                                                   int ncols)
 {
     __shared__ DD wacc[NWAVES][WMAXT];
@@ -739,8 +761,8 @@ __global__ void __launch_bounds__(BLOCK) k_wloops(Planes<T> P, Geo g, const WGro
     uint32_t base = tile_base_pair(g, p);
     uint32_t tile = base / BLOCK;
     uint32_t i = base + threadIdx.x;
-    const WGroup grp = groups[blockIdx.y / 3];
-    const int d = 1 + (int)(blockIdx.y % 3);
+    const WGroup grp = groups[group];
+    const int d = 1 + (int)blockIdx.y;
     const int R = grp.R;
     const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
     for (int t = threadIdx.x; t < NWAVES * WMAXT; t += BLOCK) wacc[t / WMAXT][t % WMAXT] = DD{0.0, 0.0};
@@ -749,7 +771,7 @@ __global__ void __launch_bounds__(BLOCK) k_wloops(Planes<T> P, Geo g, const WGro
     const bool valid = i < g.vh;
     Site s = cb_site(g, valid ? i : 0u, p);
     Site sR = shift(g, s, d, R % g.L[d]);
-    Q<T> SR = spatial_line(P, g, s, d, R);
+    Q<T> SR = link(S, g, d, s);
     Q<T> Lx = link(P, g, 0, s);
     Q<T> LxR = link(P, g, 0, sR);
     Site top = s, topR = sR;
@@ -762,7 +784,7 @@ __global__ void __launch_bounds__(BLOCK) k_wloops(Planes<T> P, Geo g, const WGro
         topR = shift(g, topR, 0, 1);
         int slot = slots[grp.slot_off + Tt - 1];
         if (slot >= 0) {
-            Q<T> St = spatial_line(P, g, top, d, R);
+            Q<T> St = link(S, g, d, top);
             T w = T(2) * qdot(qmul(Lx, St), qmul(SR, LxR));
             DD v = wave_sum_dd(DD{valid ? (double)w : 0.0, 0.0});
             if (lane == 0) wacc[wid][Tt - 1] = dd_add(wacc[wid][Tt - 1], v);

@@ -37,8 +37,9 @@ template <typename T> struct Launch {
     static hipError_t plaquette(hipStream_t s, const Geo& g, const RawPlanes& P, void* partial);
     static hipError_t ape(hipStream_t s, const Geo& g, const RawPlanes& in, const RawPlanes& out, int mode, double sp);
     // groups/slots: device tables; partial: nloop rows of 6*ceil(vh/256) columns
-    static hipError_t wloops(hipStream_t s, const Geo& g, const RawPlanes& P, const WGroupHost* groups, int ngroups,
-                             const int* slots, void* partial, int ncols);
+    static hipError_t sline(hipStream_t s, const Geo& g, const RawPlanes& P, const RawPlanes& S, int R0, int R1);
+    static hipError_t wloops(hipStream_t s, const Geo& g, const RawPlanes& P, const RawPlanes& S,
+                             const WGroupHost* groups, int group, const int* slots, void* partial, int ncols);
     // partial: 2 rows of 2*ceil(vh/256) columns (sum q, sum |q|)
     static hipError_t qtop(hipStream_t s, const Geo& g, const RawPlanes& P, int kind, void* partial,
                            double* density);

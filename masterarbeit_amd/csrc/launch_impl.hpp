@@ -118,12 +118,20 @@ hipError_t Launch<T>::ape(hipStream_t s, const Geo& g, const RawPlanes& in, cons
 }
 
 template <typename T>
-hipError_t Launch<T>::wloops(hipStream_t s, const Geo& g, const RawPlanes& P, const WGroupHost* groups, int ngroups,
-                             const int* slots, void* partial, int ncols)
+hipError_t Launch<T>::sline(hipStream_t s, const Geo& g, const RawPlanes& P, const RawPlanes& S, int R0, int R1)
 {
-    dim3 grid(2 * grid_x(g), 3 * ngroups), block(BLOCK);
-    hipLaunchKernelGGL(k_wloops<T>, grid, block, 0, s, planes<T>(P), g, (const WGroup*)groups, slots,
-                       (DD*)partial, ncols);
+    dim3 grid(2 * grid_x(g), 3), block(BLOCK);
+    hipLaunchKernelGGL(k_sline<T>, grid, block, 0, s, planes<T>(P), planes<T>(S), g, R0, R1);
+    return hipGetLastError();
+}
+
+template <typename T>
+hipError_t Launch<T>::wloops(hipStream_t s, const Geo& g, const RawPlanes& P, const RawPlanes& S,
+                             const WGroupHost* groups, int group, const int* slots, void* partial, int ncols)
+{
+    dim3 grid(2 * grid_x(g), 3), block(BLOCK);
+    hipLaunchKernelGGL(k_wloops<T>, grid, block, 0, s, planes<T>(P), planes<T>(S), g, (const WGroup*)groups, group,
+                       slots, (DD*)partial, ncols);
     return hipGetLastError();
 }
 

@@ -10,6 +10,7 @@
 #include <cstdint>
 #include <cstring>
 #include <new>
+#include <algorithm>
 #include <map>
 #include <string>
 #include <vector>
@@ -813,17 +814,23 @@ int lgt_gauge_action(lgt_engine* e, double beta, double* action)
     return LGT_OK;
 }
 
+// the second field: APE output (swapped in after every step) and, for the
+// Wilson loops, the spatial-line field
+static int ensure_alt(lgt_engine* e)
+{
+    if (e->field_alt) return LGT_OK;
+    size_t have = 0;
+    TRY(ensure(e, &e->field_alt, &have, (size_t)4 * e->vol * e->link_bytes, "APE buffer"));
+    set_planes(e->alt, e->field_alt, e->geo.vh, e->link_bytes);
+    return LGT_OK;
+}
+
 int lgt_ape(lgt_engine* e, int mode, int nsteps, double sp)
 {
     if (!e || mode < 0 || mode > 2 || nsteps < 0) return fail(LGT_EINVAL, "lgt_ape: mode %d nsteps %d", mode, nsteps);
     TRY(set_dev(e));
     if (nsteps == 0) return LGT_OK;
-    if (!e->field_alt) {
-        size_t bytes = (size_t)4 * e->vol * e->link_bytes;
-        size_t have = 0;
-        TRY(ensure(e, &e->field_alt, &have, bytes, "APE buffer"));
-        set_planes(e->alt, e->field_alt, e->geo.vh, e->link_bytes);
-    }
+    TRY(ensure_alt(e));
     int mu_lo = (mode == LGT_APE_SPATIAL) ? 1 : 0, mu_hi = (mode == LGT_APE_TEMPORAL) ? 0 : 3;
     for (int s = 0; s < nsteps; s++) {
         TRY(run(e, LGT_K_APE, "ape", [&] { return DISPATCH(e, ape(e->stream, ugeo(e, LGT_K_APE, 0), e->cur, e->alt, mode, sp)); }));
@@ -831,6 +838,26 @@ int lgt_ape(lgt_engine* e, int mode, int nsteps, double sp)
             for (int p = 0; p < 2; p++) std::swap(e->cur.p[mu][p], e->alt.p[mu][p]);
     }
     HIP_TRY(hipStreamSynchronize(e->stream), "hipStreamSynchronize");
+    return LGT_OK;
+}
+
+// the loops of `groups` (R ascending) into the partial rows: for each group
+// the line field (alt planes 1..3) is advanced to its R, then its loops run
+static int run_wloops(lgt_engine* e, const std::vector<WGroupHost>& groups, const WGroupHost* dgroups,
+                      const int* dslots, int ncols)
+{
+    int Rcur = 0;
+    for (size_t k = 0; k < groups.size(); k++) {
+        const int R = groups[k].R;
+        TRY(run(e, LGT_K_WLOOP, "spatial lines", [&] {
+            return DISPATCH(e, sline(e->stream, kgeo(e, LGT_K_WLOOP, 0), e->cur, e->alt, Rcur, R));
+        }));
+        Rcur = R;
+        TRY(run(e, LGT_K_WLOOP, "wilson loops", [&] {
+            return DISPATCH(e, wloops(e->stream, kgeo(e, LGT_K_WLOOP, 0), e->cur, e->alt, dgroups, (int)k, dslots,
+                                      e->partial, ncols));
+        }));
+    }
     return LGT_OK;
 }
 
@@ -843,7 +870,8 @@ int lgt_wilson_loops(lgt_engine* e, int nloop, const int* R, const int* T, doubl
         if (R[i] < 1) return fail(LGT_EINVAL, "lgt_wilson_loops: R = %d < 1", R[i]);
     }
     TRY(set_dev(e));
-    // unique (R,T) pairs, grouped by R in order of first appearance
+    TRY(ensure_alt(e));
+    // unique (R,T) pairs, grouped by R (ascending: the line field grows)
     std::map<std::pair<int, int>, int> uniq;
     std::vector<int> order_R;
     std::map<int, int> tmax;
@@ -859,6 +887,7 @@ int lgt_wilson_loops(lgt_engine* e, int nloop, const int* R, const int* T, doubl
         }
         tmax[R[i]] = std::max(tmax[R[i]], T[i]);
     }
+    std::sort(order_R.begin(), order_R.end());
     std::vector<WGroupHost> groups;
     std::vector<int> slots;
     for (int r : order_R) {
@@ -881,9 +910,7 @@ int lgt_wilson_loops(lgt_engine* e, int nloop, const int* R, const int* T, doubl
             "hipMemcpyAsync");
     HIP_TRY(hipMemcpyAsync(dslots, slots.data(), sizeof(int) * slots.size(), hipMemcpyHostToDevice, e->stream),
             "hipMemcpyAsync");
-    TRY(run(e, LGT_K_WLOOP, "wilson loops", [&] {
-        return DISPATCH(e, wloops(e->stream, kgeo(e, LGT_K_WLOOP, 0), e->cur, dgroups, (int)groups.size(), dslots, e->partial, ncols));
-    }));
+    TRY(run_wloops(e, groups, dgroups, dslots, ncols));
     std::vector<double> sums(nuniq);
     TRY(reduce_to_host(e, nuniq, ncols, sums.data()));
     for (int i = 0; i < nloop; i++) w[i] = sums[uniq[std::make_pair(R[i], T[i])]] / (double)(3 * e->vol * 2);
@@ -956,11 +983,7 @@ int lgt_ape_vjp_forward(lgt_engine* e, int nsteps, double sp, int segment)
     v.sp = sp;
     v.fwd = false;
     v.cot = false;
-    if (!e->field_alt) {
-        size_t have = 0;
-        TRY(ensure(e, &e->field_alt, &have, (size_t)4 * e->vol * e->link_bytes, "APE buffer"));
-        set_planes(e->alt, e->field_alt, e->geo.vh, e->link_bytes);
-    }
+    TRY(ensure_alt(e));
     // Steps before the last segment smear cur -> alt (and swap); the step just
     // before a checkpoint also keeps its |M| (the backward then emits the
     // tangent cotangent across the segment boundary itself).  The last
@@ -1011,8 +1034,8 @@ int lgt_wloop_sum(lgt_engine* e, int R, int T, double sum[2])
     int* ds = (int*)((char*)e->partial + off + sizeof(gh));
     HIP_TRY(hipMemcpyAsync(dg, &gh, sizeof(gh), hipMemcpyHostToDevice, e->stream), "hipMemcpyAsync");
     HIP_TRY(hipMemcpyAsync(ds, slots.data(), sizeof(int) * T, hipMemcpyHostToDevice, e->stream), "hipMemcpyAsync");
-    TRY(run(e, LGT_K_WLOOP, "wilson loops",
-            [&] { return DISPATCH(e, wloops(e->stream, kgeo(e, LGT_K_WLOOP, 0), e->cur, dg, 1, ds, e->partial, ncols)); }));
+    TRY(ensure_alt(e));
+    TRY(run_wloops(e, std::vector<WGroupHost>{gh}, dg, ds, ncols));
     return reduce_to_host_dd(e, 1, ncols, sum);
 }
 
