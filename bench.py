@@ -85,20 +85,38 @@ def run_chain(lgt, L, prec, seed, beta, n_or, warmup, steps, nplaq):
     }
 
 
-def time_ape(e, prec, L, nsteps=20):
+def time_ape(e, prec, L, nsteps=20, mode=0, sp=0.25):
+    """One APE step per launch; algorithmic bytes: every smeared link read and
+    written once (spatial 3 VOL, 4D 4 VOL, temporal VOL links)."""
     vol = int(np.prod(L))
-    e.set_option(1, 1)
-    e.profile_reset()
+    nlinks = {0: 3, 1: 4, 2: 1}[mode] * vol
     e.sync()
     t0 = time.perf_counter()
-    e.ape(0, nsteps, 0.25)
+    e.ape(mode, nsteps, sp)
     e.sync()
     dt = time.perf_counter() - t0
+    e.set_option(1, 1)
+    e.profile_reset()
+    e.ape(mode, nsteps, sp)
     ms, n = e.profile_read(4)
     e.set_option(1, 0)
     avg = ms / n
-    gbs = 3 * vol * APE_BYTES_PER_LINK[prec] / (avg * 1e-3) / 1e9
+    gbs = nlinks * APE_BYTES_PER_LINK[prec] / (avg * 1e-3) / 1e9
     return {"steps_per_s": nsteps / dt, "kernel_ms_avg": avg, "achieved_GBs": gbs, "frac_hbm": gbs / HBM_PEAK_GBS}
+
+
+def time_plaquette(e, prec, L, reps=20):
+    """Average plaquette: every link read once (4 links per site)."""
+    vol = int(np.prod(L))
+    e.set_option(1, 1)
+    e.profile_reset()
+    for _ in range(reps):
+        e.plaquette()
+    ms, n = e.profile_read(3)
+    e.set_option(1, 0)
+    avg = ms / n
+    gbs = vol * 4 * APE_BYTES_PER_LINK[prec] / 2 / (avg * 1e-3) / 1e9
+    return {"kernel_ms_avg": avg, "achieved_GBs": gbs, "frac_hbm": gbs / HBM_PEAK_GBS}
 
 
 def time_stream(e, prec, L, reps=10):
@@ -191,7 +209,7 @@ t0 = time.perf_counter(); ref.lib.ref_update(1); print(time.perf_counter() - t0,
 """
 
 
-def cpu_baseline(e64, L, beta, procs):
+def cpu_baseline(U, L, beta, procs):
     """The reference's own heatbath sweep (LGT_Code/modules/update/update.c:102-176,
     compiled from its sources into oracle/_ref) on the thermalised FP64
     configuration of the GPU run: once on one core (the reference is
@@ -208,7 +226,6 @@ def cpu_baseline(e64, L, beta, procs):
     vol = int(np.prod(L))
     ref = RefLib(name)
     ref.setup(beta, 1)
-    U = e64.download()
     ref.set_field(U)
     t0 = time.perf_counter()
     ref.lib.ref_update(1)
@@ -357,14 +374,21 @@ def main():
     log(f"[bench] fp32 sweep: {r32['ms_per_step']:.3f} ms/step, {r32['link_updates_per_s']:.4g} link updates/s")
     stream32 = time_stream(e32, 1, L)
     log(f"[bench] fp32 streaming pass (project): {stream32['achieved_GBs']:.0f} GB/s")
+    plaq32 = time_plaquette(e32, 1, L)
     ape32 = time_ape(e32, 1, L)
     log(f"[bench] fp32 spatial APE: {ape32['steps_per_s']:.1f} steps/s")
+    ape4d32 = time_ape(e32, 1, L, mode=1, sp=1.0 / 6.0)
+    apet32 = time_ape(e32, 1, L, mode=2)
     obs32 = time_observables(e32, L)
     e32.close()
 
     e64, r64 = run_chain(lgt, L, lgt.FP64, args.seed, args.beta, args.n_or, args.warmup, args.steps, args.nplaq)
     log(f"[bench] fp64 sweep: {r64['ms_per_step']:.3f} ms/step, {r64['link_updates_per_s']:.4g} link updates/s")
+    U64 = e64.download()   # the thermalised configuration: input of the CPU baseline
+    plaq64 = time_plaquette(e64, 0, L)
     ape64 = time_ape(e64, 0, L)
+    ape4d64 = time_ape(e64, 0, L, mode=1, sp=1.0 / 6.0)
+    apet64 = time_ape(e64, 0, L, mode=2)
     obs64 = time_observables(e64, L)
     log("[bench] APE-400 adjoint (fp64) ...")
     vjp64 = time_vjp(lgt, L, lgt.FP64, args.seed, args.beta, nsteps=args.vjp_steps)
@@ -386,7 +410,8 @@ def main():
     xfer = time_host_transfer(e64)
     if not args.no_cpu_baseline:
         log("[bench] timing the reference CPU sweep (one core, then all cores) ...")
-        base = cpu_baseline(e64, L, args.beta, args.cpu_procs)
+        base = cpu_baseline(U64, L, args.beta, args.cpu_procs)
+    del U64
     e64.close()
 
     out = {
@@ -440,6 +465,9 @@ def main():
             "n": len(p32),
         },
         "ape_spatial": {"fp32": ape32, "fp64": ape64},
+        "ape_4d": {"fp32": ape4d32, "fp64": ape4d64},
+        "ape_temporal": {"fp32": apet32, "fp64": apet64},
+        "plaquette_kernel": {"fp32": plaq32, "fp64": plaq64},
         "ape_vjp": {"fp64": vjp64, "fp32": vjp32},
         "stream_pass": stream32,
         "observables_s": {"fp32": obs32, "fp64": obs64},
