@@ -129,7 +129,7 @@ template <typename T>
 hipError_t Launch<T>::wloops(hipStream_t s, const Geo& g, const RawPlanes& P, const RawPlanes& S,
                              const WGroupHost* groups, int group, const int* slots, void* partial, int ncols)
 {
-    dim3 grid(2 * grid_x(g), 3), block(BLOCK);
+    dim3 grid(2 * grid_x(g), 3), block(WL<T>::THREADS);
     hipLaunchKernelGGL(k_wloops<T>, grid, block, 0, s, planes<T>(P), planes<T>(S), g, (const WGroup*)groups, group,
                        slots, (DD*)partial, ncols);
     return hipGetLastError();

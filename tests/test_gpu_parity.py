@@ -150,6 +150,26 @@ def test_wilson_loop_parity(lgt, L):
     np.testing.assert_array_equal(w_perm[::-1], w)
 
 
+@pytest.mark.parametrize("prec", [0, 1])
+def test_wilson_loop_unsorted_repeated_and_wrapping(lgt, prec):
+    """Loops listed in no order, repeated, with R both below and beyond the
+    spatial extent 4 (the line wraps around the lattice): each value equals
+    the oracle's for its own (R, T), and a repeated loop gets one value."""
+    L = (6, 4, 4, 8)
+    o, U = thermal_field(L)
+    o.ape(U, 0, 4, 0.25)
+    R = np.array([5, 1, 3, 5, 2, 4, 1, 6, 3])
+    T = np.array([2, 1, 5, 2, 3, 5, 5, 1, 1])
+    ref = o.wilson(U, R, T)
+    with engine(lgt, L, prec, U=U) as e:
+        w = e.wilson_loops(R, T)
+        single = np.array([e.wilson_loops([r], [t])[0] for r, t in zip(R, T)])
+    tol = dict(rtol=1e-12, atol=1e-15) if prec == 0 else dict(rtol=0, atol=2e-6)
+    np.testing.assert_allclose(w, ref, **tol)
+    np.testing.assert_array_equal(single, w)
+    assert w[0] == w[3]
+
+
 def test_wilson_loop_reference_fixture(lgt):
     for name in ["ref_l4.npz", "ref_a6448.npz"]:
         fx = load(name)
