@@ -309,6 +309,40 @@ def test_update_matches_oracle_chain(lgt, L, n_or):
     np.testing.assert_allclose(np.linalg.norm(G, axis=-1), 1.0, atol=1e-15)
 
 
+@pytest.mark.parametrize("L", [(2, 2, 2, 4), (4, 2, 6, 4), (2, 6, 2, 4)])
+def test_smallest_extents_match_oracle(lgt, L):
+    """Extent 2 (x+1 and x-1 are the same site) and lattices smaller than one
+    256-site tile: sweeps, every APE mode, plaquette, loops up to T = L0 - 1
+    with wrapping R, and both charges against the oracle."""
+    seed = 11
+    o, U = thermal_field(L, seed=4, nsweeps=1)
+    V = U.copy()
+    with engine(lgt, L, 0, seed=seed, U=U) as e:
+        acc = e.update(2, 1, 3.09, acceptance=True)
+        G = e.download()
+        acc_ref = sum(o.engine_sweep(V, 3.09, 1, seed, s) for s in range(2)) / 2
+        assert np.max(np.abs(G - V)) <= 1e-12
+        assert acc == pytest.approx(acc_ref, rel=1e-12)
+        assert e.plaquette() == pytest.approx(o.plaquette(V, acc=True), abs=1e-14)
+        R = [1, 2, 3, 1, 3]
+        T = [1, 1, 1, L[0] - 1, L[0] - 1]
+        np.testing.assert_allclose(e.wilson_loops(R, T), o.wilson(V, R, T), rtol=1e-12, atol=1e-15)
+        for mode, sp in ((0, 0.25), (1, 1.0 / 6.0), (2, 0.25)):
+            e.upload(V)
+            W = V.copy()
+            e.ape(mode, 3, sp)
+            o.ape(W, mode, 3, sp)
+            assert np.max(np.abs(e.download() - W)) <= 1e-14
+        for kind in (0, 1):
+            q, s_abs = e.qtop(kind)
+            q_ref, s_ref = o.qtop(W, "improved" if kind == 0 else "clover")[:2]
+            assert abs(q - q_ref) <= 1e-13 * s_ref
+    with engine(lgt, L, 1, seed=seed, U=V) as e32:
+        assert abs(e32.plaquette() - o.plaquette(V, acc=True)) < 1e-6
+        e32.update(2, 1, 3.09)
+        np.testing.assert_allclose(np.linalg.norm(e32.download(), axis=-1), 1.0, atol=1e-6)
+
+
 @pytest.mark.parametrize("prec", [0, 1])
 def test_results_independent_of_tile_order(lgt, prec):
     """Every tile order (XCD-contiguous, time-fastest, both, neither, each also
