@@ -12,6 +12,7 @@
 #include <new>
 #include <algorithm>
 #include <map>
+#include <thread>
 #include <string>
 #include <vector>
 
@@ -44,6 +45,12 @@ int hip_fail(hipError_t err, const char* what)
     do {                                                 \
         hipError_t _e = (call);                          \
         if (_e != hipSuccess) return hip_fail(_e, what); \
+    } while (0)
+
+#define TRY(x)                     \
+    do {                           \
+        int _rc = (x);             \
+        if (_rc != LGT_OK) return _rc; \
     } while (0)
 
 struct ProfEvent {
@@ -88,6 +95,8 @@ struct lgt_engine {
     size_t dres_bytes = 0;
     double* hres = nullptr;
     size_t hres_bytes = 0;
+    void* pin[2] = {nullptr, nullptr};           // pinned chunks of the host <-> device copies
+    hipEvent_t pin_ev[2] = {nullptr, nullptr};   // last DMA that used each chunk
     void* staging = nullptr;
     size_t staging_bytes = 0;
     int profile = 0;
@@ -140,6 +149,83 @@ int ensure_host(lgt_engine* e, size_t need)
     return LGT_OK;
 }
 
+// Host <-> device copies of whole host arrays (fields, gradients, densities).
+// A copy from or to pageable memory runs at 5-7 GB/s; here the array moves in
+// PIN_CHUNK pieces through two pinned buffers: host threads copy one piece
+// while the DMA engine moves the other, so the rate is set by the host memory
+// copy across up to PIN_THREADS cores and the link.
+constexpr size_t PIN_CHUNK = (size_t)32 << 20;
+constexpr unsigned PIN_THREADS = 8;
+
+void par_copy(void* dst, const void* src, size_t n)
+{
+    const unsigned hw = std::max(1u, std::thread::hardware_concurrency());
+    const unsigned nt = (unsigned)std::min<size_t>({(size_t)PIN_THREADS, (size_t)hw, n / ((size_t)1 << 20) + 1});
+    if (nt <= 1) {
+        memcpy(dst, src, n);
+        return;
+    }
+    const size_t part = (n / nt + 63) & ~(size_t)63;
+    std::vector<std::thread> pool;
+    for (unsigned t = 1; t < nt; t++) {
+        // the last thread also takes what the rounded-down parts leave
+        const size_t off = std::min(n, part * t), len = (t + 1 == nt) ? n - off : std::min(n - off, part);
+        if (len) pool.emplace_back([=] { memcpy((char*)dst + off, (const char*)src + off, len); });
+    }
+    memcpy(dst, src, std::min(n, part));
+    for (auto& th : pool) th.join();
+}
+
+int ensure_pinned(lgt_engine* e)
+{
+    for (int b = 0; b < 2; b++) {
+        if (!e->pin[b]) HIP_TRY(hipHostMalloc(&e->pin[b], PIN_CHUNK, hipHostMallocDefault), "hipHostMalloc");
+        if (!e->pin_ev[b]) HIP_TRY(hipEventCreateWithFlags(&e->pin_ev[b], hipEventDisableTiming), "hipEventCreate");
+    }
+    return LGT_OK;
+}
+
+// enqueues host -> device on the engine's stream; returns when the host array
+// has been read (the last DMA may still run on the stream)
+int h2d(lgt_engine* e, void* dst, const void* src, size_t bytes)
+{
+    TRY(ensure_pinned(e));
+    for (size_t off = 0, c = 0; off < bytes; off += PIN_CHUNK, c++) {
+        const int b = (int)(c & 1);
+        const size_t n = std::min(PIN_CHUNK, bytes - off);
+        // the chunk's last DMA (of this or an earlier copy) has read it
+        HIP_TRY(hipEventSynchronize(e->pin_ev[b]), "hipEventSynchronize");
+        par_copy(e->pin[b], (const char*)src + off, n);
+        HIP_TRY(hipMemcpyAsync((char*)dst + off, e->pin[b], n, hipMemcpyHostToDevice, e->stream), "hipMemcpyAsync H2D");
+        HIP_TRY(hipEventRecord(e->pin_ev[b], e->stream), "hipEventRecord");
+    }
+    return LGT_OK;
+}
+
+// device -> host after the work already queued on the engine's stream;
+// returns with the host array complete
+int d2h(lgt_engine* e, void* dst, const void* src, size_t bytes)
+{
+    TRY(ensure_pinned(e));
+    const size_t nch = (bytes + PIN_CHUNK - 1) / PIN_CHUNK;
+    auto issue = [&](size_t c) -> int {
+        const int b = (int)(c & 1);
+        const size_t off = c * PIN_CHUNK, n = std::min(PIN_CHUNK, bytes - off);
+        HIP_TRY(hipMemcpyAsync(e->pin[b], (const char*)src + off, n, hipMemcpyDeviceToHost, e->stream), "hipMemcpyAsync D2H");
+        HIP_TRY(hipEventRecord(e->pin_ev[b], e->stream), "hipEventRecord");
+        return LGT_OK;
+    };
+    if (nch) TRY(issue(0));
+    for (size_t c = 0; c < nch; c++) {
+        const int b = (int)(c & 1);
+        const size_t off = c * PIN_CHUNK, n = std::min(PIN_CHUNK, bytes - off);
+        if (c + 1 < nch) TRY(issue(c + 1));   // the other chunk was emptied by the last pass
+        HIP_TRY(hipEventSynchronize(e->pin_ev[b]), "hipEventSynchronize");
+        par_copy((char*)dst + off, e->pin[b], n);
+    }
+    return LGT_OK;
+}
+
 void set_planes(RawPlanes& P, void* base, uint32_t vh, size_t link_bytes)
 {
     for (int mu = 0; mu < 4; mu++)
@@ -182,12 +268,6 @@ int flush_profile(lgt_engine* e)
     e->events.clear();
     return LGT_OK;
 }
-
-#define TRY(x)                     \
-    do {                           \
-        int _rc = (x);             \
-        if (_rc != LGT_OK) return _rc; \
-    } while (0)
 
 #define DISPATCH(e, call) ((e)->prec == LGT_FP64 ? Launch<double>::call : Launch<float>::call)
 
@@ -464,6 +544,10 @@ int lgt_destroy(lgt_engine* e)
     if (e->dres) hipFree(e->dres);
     if (e->staging) hipFree(e->staging);
     if (e->hres) hipHostFree(e->hres);
+    for (int b = 0; b < 2; b++) {
+        if (e->pin[b]) hipHostFree(e->pin[b]);
+        if (e->pin_ev[b]) hipEventDestroy(e->pin_ev[b]);
+    }
     if (e->stream) hipStreamDestroy(e->stream);
     delete e;
     return LGT_OK;
@@ -518,7 +602,7 @@ int lgt_upload(lgt_engine* e, const double* aos)
     TRY(set_dev(e));
     size_t bytes = sizeof(double) * 16 * (size_t)e->vol;
     TRY(ensure(e, &e->staging, &e->staging_bytes, bytes, "staging"));
-    HIP_TRY(hipMemcpyAsync(e->staging, aos, bytes, hipMemcpyHostToDevice, e->stream), "hipMemcpyAsync H2D");
+    TRY(h2d(e, e->staging, aos, bytes));
     TRY(run(e, LGT_K_FIELD, "from_aos",
             [&] { return DISPATCH(e, from_aos(e->stream, e->geo, e->cur, (const double*)e->staging)); }));
     HIP_TRY(hipStreamSynchronize(e->stream), "hipStreamSynchronize");
@@ -532,9 +616,7 @@ int lgt_download(lgt_engine* e, double* aos)
     size_t bytes = sizeof(double) * 16 * (size_t)e->vol;
     TRY(ensure(e, &e->staging, &e->staging_bytes, bytes, "staging"));
     TRY(run(e, LGT_K_FIELD, "to_aos", [&] { return DISPATCH(e, to_aos(e->stream, e->geo, e->cur, (double*)e->staging)); }));
-    HIP_TRY(hipMemcpyAsync(aos, e->staging, bytes, hipMemcpyDeviceToHost, e->stream), "hipMemcpyAsync D2H");
-    HIP_TRY(hipStreamSynchronize(e->stream), "hipStreamSynchronize");
-    return LGT_OK;
+    return d2h(e, aos, e->staging, bytes);
 }
 
 // ---- configuration files (config_IO.c:40-117,537-621) ----
@@ -937,7 +1019,7 @@ int lgt_qtop(lgt_engine* e, int kind, double* q, double* abs_sum, double* densit
     *q = s[0] / norm;
     if (abs_sum) *abs_sum = s[1] / norm;
     if (density) {
-        HIP_TRY(hipMemcpy(density, dq, sizeof(double) * (size_t)e->vol, hipMemcpyDeviceToHost), "hipMemcpy D2H");
+        TRY(d2h(e, density, dq, sizeof(double) * (size_t)e->vol));
     }
     return LGT_OK;
 }
@@ -1044,9 +1126,7 @@ static int field_to_host(lgt_engine* e, const RawPlanes& P, double* aos)
     size_t bytes = sizeof(double) * 16 * (size_t)e->vol;
     TRY(ensure(e, &e->staging, &e->staging_bytes, bytes, "staging"));
     TRY(run(e, LGT_K_FIELD, "to_aos", [&] { return DISPATCH(e, to_aos(e->stream, e->geo, P, (double*)e->staging)); }));
-    HIP_TRY(hipMemcpyAsync(aos, e->staging, bytes, hipMemcpyDeviceToHost, e->stream), "hipMemcpyAsync D2H");
-    HIP_TRY(hipStreamSynchronize(e->stream), "hipStreamSynchronize");
-    return LGT_OK;
+    return d2h(e, aos, e->staging, bytes);
 }
 
 int lgt_wloop_grad(lgt_engine* e, int R, int T, double* grad)
@@ -1074,7 +1154,7 @@ int lgt_ape_vjp_backward(lgt_engine* e, const double* cotangent, double* grad)
         if (!v.gobj) TRY(alloc_field(e, &v.gobj, &v.gobjP));
         size_t bytes = sizeof(double) * 16 * (size_t)e->vol;
         TRY(ensure(e, &e->staging, &e->staging_bytes, bytes, "staging"));
-        HIP_TRY(hipMemcpyAsync(e->staging, cotangent, bytes, hipMemcpyHostToDevice, e->stream), "hipMemcpyAsync H2D");
+        TRY(h2d(e, e->staging, cotangent, bytes));
         TRY(run(e, LGT_K_FIELD, "from_aos",
                 [&] { return DISPATCH(e, from_aos(e->stream, e->geo, v.gobjP, (const double*)e->staging)); }));
         v.cot = true;
