@@ -170,7 +170,12 @@ void par_copy(void* dst, const void* src, size_t n)
     for (unsigned t = 1; t < nt; t++) {
         // the last thread also takes what the rounded-down parts leave
         const size_t off = std::min(n, part * t), len = (t + 1 == nt) ? n - off : std::min(n - off, part);
-        if (len) pool.emplace_back([=] { memcpy((char*)dst + off, (const char*)src + off, len); });
+        if (!len) continue;
+        try {
+            pool.emplace_back([=] { memcpy((char*)dst + off, (const char*)src + off, len); });
+        } catch (...) {   // no thread to be had: this part in the calling thread
+            memcpy((char*)dst + off, (const char*)src + off, len);
+        }
     }
     memcpy(dst, src, std::min(n, part));
     for (auto& th : pool) th.join();
