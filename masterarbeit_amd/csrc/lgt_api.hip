@@ -101,8 +101,8 @@ struct lgt_engine {
     size_t staging_bytes = 0;
     int profile = 0;
     int order = -1;   // tile order of every kernel; -1: per kernel (see kgeo)
-    int alternate = 1;  // update launches alternate between forward and reversed tile order
-    uint32_t flip = 0;  // reverses the next update launch
+    int alternate = 1;  // chain launches alternate between forward and reversed tile order
+    uint32_t flip = 0;  // reverses the next chain launch (see ugeo)
     std::vector<ProfEvent> events;
     double prof_ms[LGT_K_COUNT] = {0};
     int64_t prof_n[LGT_K_COUNT] = {0};
@@ -150,7 +150,7 @@ int ensure_host(lgt_engine* e, size_t need)
 }
 
 // Host <-> device copies of whole host arrays (fields, gradients, densities).
-// A copy from or to pageable memory runs at 5-7 GB/s; here the array moves in
+// A copy from or to pageable memory ran at 10-29 GB/s; here the array moves in
 // PIN_CHUNK pieces through two pinned buffers: host threads copy one piece
 // while the DMA engine moves the other, so the rate is set by the host memory
 // copy across up to PIN_THREADS cores and the link.
