@@ -69,10 +69,15 @@ def run_chain(lgt, L, prec, seed, beta, n_or, warmup, steps, nplaq):
         if n:
             kern[name] = {"ms_total": ms, "launches": n, "ms_avg": ms / n}
     e.set_option(lgt.engine.OPT_PROFILE, 0)
+    # the reference main's loop (qcd.c:53-59): one sweep call, then the
+    # plaquette, per configuration
     plaqs = []
+    e.sync()
+    tq = time.perf_counter()
     for _ in range(nplaq):
         e.update(1, n_or, beta)
         plaqs.append(e.plaquette())
+    tq = time.perf_counter() - tq
     dt = t1 - t0
     return e, {
         "seconds": dt,
@@ -81,6 +86,7 @@ def run_chain(lgt, L, prec, seed, beta, n_or, warmup, steps, nplaq):
         "link_updates_per_s": 4.0 * vol * steps / dt,
         "kernels": kern,
         "plaq": plaqs,
+        "qcd_loop_ms_per_step": 1e3 * tq / nplaq if nplaq else None,
         "device_bytes": e.device_bytes(),
     }
 
@@ -451,8 +457,10 @@ def main():
             "link_updates_per_s": r64["link_updates_per_s"],
             "ms_per_step": r64["ms_per_step"],
             "ms_per_step_profiled": r64["ms_per_step_profiled"],
+            "qcd_loop_ms_per_step": r64["qcd_loop_ms_per_step"],
             "kernels": r64["kernels"],
         },
+        "fp32_qcd_loop_ms_per_step": r32["qcd_loop_ms_per_step"],
         "fp32_over_fp64": r32["link_updates_per_s"] / r64["link_updates_per_s"],
         "fp32_kernels": r32["kernels"],
         "fp32_ms_per_step_profiled": r32["ms_per_step_profiled"],
