@@ -194,21 +194,25 @@ int lgt_wilson_loops(lgt_engine *e, int nloop, const int *R, const int *T, doubl
  * (nullable): q(x) per site in lexicographic order, VOL doubles. */
 int lgt_qtop(lgt_engine *e, int kind, double *q, double *abs_sum, double *density);
 
-/* ---- adjoint of spatial APE smearing (no reference counterpart) -----------
- * A forward/backward pair for the vector-Jacobian product of N spatial APE
- * steps (smearing_APE_spatial, LGT_Code/modules/obs/smearing.c:107-138).
+/* ---- adjoint of APE smearing (no reference counterpart) -------------------
+ * A forward/backward pair for the vector-Jacobian product of N APE steps of
+ * a mode (LGT_APE_SPATIAL, LGT_APE_ALL, LGT_APE_TEMPORAL: smearing_APE_*,
+ * LGT_Code/modules/obs/smearing.c:74-168).
  *
- * lgt_ape_vjp_forward smears the current field like lgt_ape(LGT_APE_SPATIAL,
- * nsteps, sp) and keeps a checkpoint of the spatial links every `segment`
+ * lgt_ape_vjp_forward smears the current field like lgt_ape(mode, nsteps,
+ * sp) and keeps a checkpoint of the links the mode changes every `segment`
  * steps (0: ceil(sqrt(nsteps))).  lgt_ape_vjp_backward then maps a cotangent
  * dL/dU of the smeared field to dL/dU of the thin field, recomputing each
  * segment's steps from its checkpoint (storing every step's |U + sp S|) and
  * running the adjoint steps backwards through it.  Cotangents and gradients
  * have 4 real components per link, Euclidean (<a,b> = a0 b0 + a.b), in the
- * field layout of lgt_upload; the temporal links, untouched by spatial
- * smearing, pass their cotangent through.  cotangent may be NULL to use the
- * one left on the device by lgt_wloop_grad.  The smeared field stays current. */
-int lgt_ape_vjp_forward(lgt_engine *e, int nsteps, double sp, int segment);
+ * field layout of lgt_upload; links the mode does not smear keep their
+ * cotangent and add what they contribute to the smeared links' staples (the
+ * temporal links of spatial smearing, in no staple, pass it through).
+ * cotangent may be NULL to use the one left on the device by lgt_wloop_grad.
+ * The smeared field stays current; it must not be changed between the two
+ * calls. */
+int lgt_ape_vjp_forward(lgt_engine *e, int mode, int nsteps, double sp, int segment);
 int lgt_ape_vjp_backward(lgt_engine *e, const double *cotangent, double *grad);
 /* The R x T loops of measureWilsonLoop (LGT_Code/modules/obs/wilson.c:21-56)
  * summed without normalisation, L = sum_x sum_{d=1..3} 2 Re[W(x; t, d)]_0,

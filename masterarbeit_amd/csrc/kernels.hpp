@@ -648,8 +648,9 @@ __device__ __forceinline__ Q<T> ape_staple(const Planes<T>& Z, const Geo& g, con
 // Z_a(n-a+b)) are loaded once: 24 loads for the 3 spatial links instead of
 // 36, 44 instead of 72 for all four.  Each staple is still summed in the
 // reference's order (k ascending, forward then backward term).
-// grid: x over (tile, parity) pairs.  With Mn.p[1][0] set (spatial mode),
-// |M| = |Z + sp S| of every smeared link is stored too (for the adjoint).
+// grid: x over (tile, parity) pairs.  With Mn's planes of the smeared
+// directions set, |M| = |Z + sp S| of every smeared link is stored too (for
+// the adjoint).
 template <typename T> struct NormPlanes { T* p[4][2]; };
 
 template <typename T, int MODE>
@@ -700,7 +701,7 @@ __global__ void __launch_bounds__(BLOCK) k_ape(Planes<T> Z, Planes<T> O, Geo g, 
     // stores last: no store separates the loads the directions share
 #pragma unroll
     for (int mu = MU0; mu <= MU1; mu++) {
-        if (MODE == 0 && Mn.p[1][0]) Mn.p[mu][p][i] = nrm[mu];
+        if (Mn.p[MU0][0]) Mn.p[mu][p][i] = nrm[mu];
         qstore<T>(O.p[mu][p], i, res[mu]);
     }
 }

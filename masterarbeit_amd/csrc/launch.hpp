@@ -43,13 +43,13 @@ template <typename T> struct Launch {
     // partial: 2 rows of 2*ceil(vh/256) columns (sum q, sum |q|)
     static hipError_t qtop(hipStream_t s, const Geo& g, const RawPlanes& P, int kind, void* partial,
                            double* density);
-    // adjoint of spatial APE (vjp_kernels.hpp)
+    // adjoint of APE, mode 0 spatial / 1 all / 2 temporal (vjp_kernels.hpp)
     static hipError_t ape_save(hipStream_t s, const Geo& g, const RawPlanes& in, const RawPlanes& out,
-                               const RawNorms& mn, double sp);
+                               const RawNorms& mn, int mode, double sp);
     static hipError_t vjp_tangent(hipStream_t s, const Geo& g, const RawPlanes& U, const RawPlanes& G,
-                                  const RawNorms& mn, const RawPlanes& H);
+                                  const RawNorms& mn, const RawPlanes& H, int mode);
     static hipError_t vjp_step(hipStream_t s, const Geo& g, const RawPlanes& Z, const RawPlanes& Hin,
-                               const RawNorms* mprev, const RawPlanes& out, double sp);
+                               const RawNorms* mprev, const RawPlanes& out, int mode, double sp);
     static hipError_t wloop_grad(hipStream_t s, const Geo& g, const RawPlanes& U, int R, int Tt, const RawPlanes& G);
 };
 

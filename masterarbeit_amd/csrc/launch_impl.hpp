@@ -156,29 +156,41 @@ static NormPlanes<T> norms(const RawNorms& R)
 
 template <typename T>
 hipError_t Launch<T>::ape_save(hipStream_t s, const Geo& g, const RawPlanes& in, const RawPlanes& out,
-                               const RawNorms& mn, double sp)
+                               const RawNorms& mn, int mode, double sp)
 {
-    hipLaunchKernelGGL((k_ape<T, 0>), dim3(2 * grid_x(g)), dim3(BLOCK), 0, s, planes<T>(in), planes<T>(out), g, (T)sp,
-                       norms<T>(mn));
+    dim3 grid(2 * grid_x(g)), block(BLOCK);
+    Planes<T> Z = planes<T>(in), O = planes<T>(out);
+    if (mode == 0) hipLaunchKernelGGL((k_ape<T, 0>), grid, block, 0, s, Z, O, g, (T)sp, norms<T>(mn));
+    else if (mode == 1) hipLaunchKernelGGL((k_ape<T, 1>), grid, block, 0, s, Z, O, g, (T)sp, norms<T>(mn));
+    else hipLaunchKernelGGL((k_ape<T, 2>), grid, block, 0, s, Z, O, g, (T)sp, norms<T>(mn));
     return hipGetLastError();
 }
 
 template <typename T>
 hipError_t Launch<T>::vjp_tangent(hipStream_t s, const Geo& g, const RawPlanes& U, const RawPlanes& G,
-                                  const RawNorms& mn, const RawPlanes& H)
+                                  const RawNorms& mn, const RawPlanes& H, int mode)
 {
-    hipLaunchKernelGGL(k_vjp_tangent<T>, dim3(grid_x(g), 6), dim3(BLOCK), 0, s, planes<T>(U), planes<T>(G), norms<T>(mn),
-                       planes<T>(H), g);
+    const dim3 block(BLOCK);
+    const Planes<T> Up = planes<T>(U), Gp = planes<T>(G), Hp = planes<T>(H);
+    const NormPlanes<T> M = norms<T>(mn);
+    if (mode == 0) hipLaunchKernelGGL((k_vjp_tangent<T, 0>), dim3(grid_x(g), 6), block, 0, s, Up, Gp, M, Hp, g);
+    else if (mode == 1) hipLaunchKernelGGL((k_vjp_tangent<T, 1>), dim3(grid_x(g), 8), block, 0, s, Up, Gp, M, Hp, g);
+    else hipLaunchKernelGGL((k_vjp_tangent<T, 2>), dim3(grid_x(g), 8), block, 0, s, Up, Gp, M, Hp, g);
     return hipGetLastError();
 }
 
 template <typename T>
 hipError_t Launch<T>::vjp_step(hipStream_t s, const Geo& g, const RawPlanes& Z, const RawPlanes& Hin,
-                               const RawNorms* mprev, const RawPlanes& out, double sp)
+                               const RawNorms* mprev, const RawPlanes& out, int mode, double sp)
 {
     RawNorms none{};
-    hipLaunchKernelGGL(k_vjp_step<T>, dim3(2 * grid_x(g)), dim3(BLOCK), 0, s, planes<T>(Z), planes<T>(Hin),
-                       norms<T>(mprev ? *mprev : none), mprev ? 1 : 0, planes<T>(out), g, (T)sp);
+    const dim3 grid(2 * grid_x(g)), block(BLOCK);
+    const Planes<T> Zp = planes<T>(Z), Hp = planes<T>(Hin), Op = planes<T>(out);
+    const NormPlanes<T> M = norms<T>(mprev ? *mprev : none);
+    const int hp = mprev ? 1 : 0;
+    if (mode == 0) hipLaunchKernelGGL((k_vjp_step<T, 0>), grid, block, 0, s, Zp, Hp, M, hp, Op, g, (T)sp);
+    else if (mode == 1) hipLaunchKernelGGL((k_vjp_step<T, 1>), grid, block, 0, s, Zp, Hp, M, hp, Op, g, (T)sp);
+    else hipLaunchKernelGGL((k_vjp_step<T, 2>), grid, block, 0, s, Zp, Hp, M, hp, Op, g, (T)sp);
     return hipGetLastError();
 }
 

@@ -61,15 +61,18 @@ struct ProfEvent {
 // device state of the APE adjoint: checkpoints, one segment of recomputed
 // steps with their |M|, two cotangent buffers and the objective's gradient
 struct VjpState {
-    int nsteps = 0, seg = 0, nck = 0;
+    int mode = 0, nsteps = 0, seg = 0, nck = 0;
     double sp = 0.0;
     bool fwd = false, cot = false;
     bool top_saved = false;   // segment buffers hold the forward's last segment
+    size_t fbytes = 0, nbytes = 0, gbytes = 0;   // one field / norm / cotangent buffer of the layout
     std::vector<void*> ck, ckM, segU, segM;
     std::vector<RawPlanes> ckP, segUP;
     std::vector<RawNorms> ckMN, segMN;   // ckMN[c]: |M| of the step before checkpoint c (c >= 1)
     void* g[2] = {nullptr, nullptr};
     RawPlanes gP[2];
+    void* fixed = nullptr;   // temporal mode: the spatial links, which the smearing leaves alone
+    RawPlanes fixedP;
     void* gobj = nullptr;
     RawPlanes gobjP;
 };
@@ -373,31 +376,41 @@ int alloc_field(lgt_engine* e, void** base, RawPlanes* P)
     return LGT_OK;
 }
 
-// Buffers of the smearing adjoint hold the spatial links only (the temporal
-// links neither change under spatial APE nor enter its staples): planes
-// [mu][p] for mu = 1..3 at ((mu-1)*2 + p) * VOL/2, p[0][*] null.
-size_t spatial_field_bytes(const lgt_engine* e) { return (size_t)3 * e->vol * e->link_bytes; }
-size_t spatial_norm_bytes(const lgt_engine* e) { return (size_t)3 * e->vol * (e->link_bytes / 4); }
+// Buffers of the smearing adjoint hold only the directions they need: the
+// fields of the steps the links the mode smears (spatial 1..3, all 0..3,
+// temporal 0; the spatial links of temporal smearing, which never change, are
+// kept once in `fixed`), |M| those same links, the cotangents every link a
+// staple touches (spatial 1..3: the temporal links are in no staple; else
+// 0..3).  Planes [mu][p] of a buffer for mu in [lo, hi] lie at
+// ((mu - lo) * 2 + p) * VOL/2 elements; the others are null (or `fixed`).
+struct DirRange { int lo, hi; };
+DirRange vjp_link_dirs(int mode) { return mode == 0 ? DirRange{1, 3} : (mode == 1 ? DirRange{0, 3} : DirRange{0, 0}); }
+DirRange vjp_cot_dirs(int mode) { return mode == 0 ? DirRange{1, 3} : DirRange{0, 3}; }
 
-int alloc_spatial_field(lgt_engine* e, void** base, RawPlanes* P)
+size_t dir_bytes(const lgt_engine* e, DirRange r, size_t el) { return (size_t)(r.hi - r.lo + 1) * e->vol * el; }
+
+template <class P>
+int alloc_dirs(lgt_engine* e, DirRange r, size_t el, void** base, P* out, const char* what)
 {
     size_t have = 0;
     *base = nullptr;
-    TRY(ensure(e, base, &have, spatial_field_bytes(e), "adjoint field"));
-    for (int p = 0; p < 2; p++) P->p[0][p] = nullptr;
-    for (int mu = 1; mu < 4; mu++)
-        for (int p = 0; p < 2; p++) P->p[mu][p] = (char*)*base + ((size_t)((mu - 1) * 2 + p) * e->geo.vh) * e->link_bytes;
+    TRY(ensure(e, base, &have, dir_bytes(e, r, el), what));
+    for (int mu = 0; mu < 4; mu++)
+        for (int p = 0; p < 2; p++)
+            out->p[mu][p] = (mu >= r.lo && mu <= r.hi)
+                                ? (void*)((char*)*base + ((size_t)((mu - r.lo) * 2 + p) * e->geo.vh) * el)
+                                : nullptr;
     return LGT_OK;
 }
 
-int alloc_spatial_norms(lgt_engine* e, void** base, RawNorms* N)
+// a field buffer of the mode's smeared links; in temporal mode its spatial
+// planes are those of `fixed`
+int alloc_vjp_field(lgt_engine* e, int mode, void** base, RawPlanes* P)
 {
-    size_t el = e->link_bytes / 4, have = 0;
-    *base = nullptr;
-    TRY(ensure(e, base, &have, spatial_norm_bytes(e), "adjoint norms"));
-    for (int p = 0; p < 2; p++) N->p[0][p] = nullptr;
-    for (int mu = 1; mu < 4; mu++)
-        for (int p = 0; p < 2; p++) N->p[mu][p] = (char*)*base + ((size_t)((mu - 1) * 2 + p) * e->geo.vh) * el;
+    TRY(alloc_dirs(e, vjp_link_dirs(mode), e->link_bytes, base, P, "adjoint field"));
+    if (mode == 2)
+        for (int mu = 1; mu < 4; mu++)
+            for (int p = 0; p < 2; p++) P->p[mu][p] = e->vjp.fixedP.p[mu][p];
     return LGT_OK;
 }
 
@@ -413,16 +426,16 @@ void free_buf(lgt_engine* e, void*& b, size_t bytes)
 void free_vjp(lgt_engine* e, bool keep_cot)
 {
     VjpState& v = e->vjp;
-    size_t fb = spatial_field_bytes(e), nb = spatial_norm_bytes(e);
-    for (auto& b : v.ck) free_buf(e, b, fb);
-    for (auto& b : v.ckM) free_buf(e, b, nb);
-    for (auto& b : v.segU) free_buf(e, b, fb);
-    for (auto& b : v.segM) free_buf(e, b, nb);
+    for (auto& b : v.ck) free_buf(e, b, v.fbytes);
+    for (auto& b : v.ckM) free_buf(e, b, v.nbytes);
+    for (auto& b : v.segU) free_buf(e, b, v.fbytes);
+    for (auto& b : v.segM) free_buf(e, b, v.nbytes);
     v.ck.clear(); v.ckM.clear(); v.segU.clear(); v.segM.clear();
     v.ckP.clear(); v.ckMN.clear(); v.segUP.clear(); v.segMN.clear();
     v.top_saved = false;
-    free_buf(e, v.g[0], fb);
-    free_buf(e, v.g[1], fb);
+    free_buf(e, v.g[0], v.gbytes);
+    free_buf(e, v.g[1], v.gbytes);
+    free_buf(e, v.fixed, dir_bytes(e, DirRange{1, 3}, e->link_bytes));
     if (!keep_cot) {
         free_buf(e, v.gobj, (size_t)4 * e->vol * e->link_bytes);
         v.cot = false;
@@ -431,10 +444,11 @@ void free_vjp(lgt_engine* e, bool keep_cot)
     v.nsteps = v.seg = v.nck = 0;
 }
 
-int copy_spatial(lgt_engine* e, const RawPlanes& dst, const RawPlanes& src)
+// copies the planes of directions r from src to dst
+int copy_dirs(lgt_engine* e, const RawPlanes& dst, const RawPlanes& src, DirRange r)
 {
     size_t bytes = (size_t)e->geo.vh * e->link_bytes;
-    for (int mu = 1; mu < 4; mu++)
+    for (int mu = r.lo; mu <= r.hi; mu++)
         for (int p = 0; p < 2; p++)
             HIP_TRY(hipMemcpyAsync(dst.p[mu][p], src.p[mu][p], bytes, hipMemcpyDeviceToDevice, e->stream),
                     "hipMemcpyAsync D2D");
@@ -1029,35 +1043,43 @@ int lgt_qtop(lgt_engine* e, int kind, double* q, double* abs_sum, double* densit
     return LGT_OK;
 }
 
-int lgt_ape_vjp_forward(lgt_engine* e, int nsteps, double sp, int segment)
+int lgt_ape_vjp_forward(lgt_engine* e, int mode, int nsteps, double sp, int segment)
 {
-    if (!e || nsteps < 1 || segment < 0) return fail(LGT_EINVAL, "lgt_ape_vjp_forward: nsteps %d segment %d", nsteps, segment);
+    if (!e || nsteps < 1 || segment < 0 || mode < 0 || mode > 2)
+        return fail(LGT_EINVAL, "lgt_ape_vjp_forward: mode %d nsteps %d segment %d", mode, nsteps, segment);
     TRY(set_dev(e));
     VjpState& v = e->vjp;
     int seg = segment > 0 ? segment : (int)std::ceil(std::sqrt((double)nsteps));
     if (seg > nsteps) seg = nsteps;
     const int nck = (nsteps + seg - 1) / seg;
+    const DirRange lr = vjp_link_dirs(mode), cr = vjp_cot_dirs(mode);
     // checkpoints, segment fields and norms, cotangents: allocated here once
-    // for a layout (seg, nck) and kept for later calls with the same layout,
-    // so neither pass allocates when the adjoint is repeated
-    if (v.seg != seg || v.nck != nck) {
+    // for a layout (mode, seg, nck) and kept for later calls with the same
+    // layout, so neither pass allocates when the adjoint is repeated
+    if (v.seg != seg || v.nck != nck || v.mode != mode) {
         free_vjp(e, true);
+        v.mode = mode;
+        v.fbytes = dir_bytes(e, lr, e->link_bytes);
+        v.nbytes = dir_bytes(e, lr, e->link_bytes / 4);
+        v.gbytes = dir_bytes(e, cr, e->link_bytes);
+        int rc = LGT_OK;
+        if (mode == 2) rc = alloc_dirs(e, DirRange{1, 3}, e->link_bytes, &v.fixed, &v.fixedP, "adjoint field");
         v.ck.assign(nck, nullptr);
         v.ckP.resize(nck);
         v.ckM.assign(nck, nullptr);
         v.ckMN.resize(nck);
-        int rc = LGT_OK;
-        for (int c = 0; c < nck && rc == LGT_OK; c++) rc = alloc_spatial_field(e, &v.ck[c], &v.ckP[c]);
-        for (int c = 1; c < nck && rc == LGT_OK; c++) rc = alloc_spatial_norms(e, &v.ckM[c], &v.ckMN[c]);
-        if (rc == LGT_OK) rc = alloc_spatial_field(e, &v.g[0], &v.gP[0]);
-        if (rc == LGT_OK) rc = alloc_spatial_field(e, &v.g[1], &v.gP[1]);
+        for (int c = 0; c < nck && rc == LGT_OK; c++) rc = alloc_vjp_field(e, mode, &v.ck[c], &v.ckP[c]);
+        for (int c = 1; c < nck && rc == LGT_OK; c++)
+            rc = alloc_dirs(e, lr, e->link_bytes / 4, &v.ckM[c], &v.ckMN[c], "adjoint norms");
+        if (rc == LGT_OK) rc = alloc_dirs(e, cr, e->link_bytes, &v.g[0], &v.gP[0], "adjoint cotangent");
+        if (rc == LGT_OK) rc = alloc_dirs(e, cr, e->link_bytes, &v.g[1], &v.gP[1], "adjoint cotangent");
         v.segU.assign(seg, nullptr);
         v.segUP.resize(seg);
         v.segM.assign(seg, nullptr);
         v.segMN.resize(seg);
         for (int j = 0; j < seg && rc == LGT_OK; j++) {
-            rc = alloc_spatial_field(e, &v.segU[j], &v.segUP[j]);
-            if (rc == LGT_OK) rc = alloc_spatial_norms(e, &v.segM[j], &v.segMN[j]);
+            rc = alloc_vjp_field(e, mode, &v.segU[j], &v.segUP[j]);
+            if (rc == LGT_OK) rc = alloc_dirs(e, lr, e->link_bytes / 4, &v.segM[j], &v.segMN[j], "adjoint norms");
         }
         if (rc != LGT_OK) {
             free_vjp(e, true);
@@ -1071,6 +1093,7 @@ int lgt_ape_vjp_forward(lgt_engine* e, int nsteps, double sp, int segment)
     v.fwd = false;
     v.cot = false;
     TRY(ensure_alt(e));
+    if (mode == 2) TRY(copy_dirs(e, v.fixedP, e->cur, DirRange{1, 3}));
     // Steps before the last segment smear cur -> alt (and swap); the step just
     // before a checkpoint also keeps its |M| (the backward then emits the
     // tangent cotangent across the segment boundary itself).  The last
@@ -1079,24 +1102,26 @@ int lgt_ape_vjp_forward(lgt_engine* e, int nsteps, double sp, int segment)
     const int n0_last = (nck - 1) * seg;
     v.top_saved = false;
     for (int n = 0; n < nsteps; n++) {
-        if (n % seg == 0) TRY(copy_spatial(e, v.ckP[n / seg], e->cur));
+        if (n % seg == 0) TRY(copy_dirs(e, v.ckP[n / seg], e->cur, lr));
         if (n < n0_last) {
             if ((n + 1) % seg == 0) {
                 TRY(run(e, LGT_K_APE, "ape", [&] {
-                    return DISPATCH(e, ape_save(e->stream, ugeo(e, LGT_K_APE, 0), e->cur, e->alt, v.ckMN[(n + 1) / seg], sp));
+                    return DISPATCH(e, ape_save(e->stream, ugeo(e, LGT_K_APE, 0), e->cur, e->alt, v.ckMN[(n + 1) / seg],
+                                                mode, sp));
                 }));
             } else {
-                TRY(run(e, LGT_K_APE, "ape", [&] { return DISPATCH(e, ape(e->stream, ugeo(e, LGT_K_APE, 0), e->cur, e->alt, 0, sp)); }));
+                TRY(run(e, LGT_K_APE, "ape",
+                        [&] { return DISPATCH(e, ape(e->stream, ugeo(e, LGT_K_APE, 0), e->cur, e->alt, mode, sp)); }));
             }
         } else {
             const RawPlanes& in = (n == n0_last) ? e->cur : v.segUP[n - n0_last - 1];
             const RawPlanes& out = (n == nsteps - 1) ? e->alt : v.segUP[n - n0_last];
             TRY(run(e, LGT_K_APE, "ape", [&] {
-                return DISPATCH(e, ape_save(e->stream, ugeo(e, LGT_K_APE, 0), in, out, v.segMN[n - n0_last], sp));
+                return DISPATCH(e, ape_save(e->stream, ugeo(e, LGT_K_APE, 0), in, out, v.segMN[n - n0_last], mode, sp));
             }));
         }
         if (n < n0_last || n == nsteps - 1)
-            for (int mu = 1; mu < 4; mu++)
+            for (int mu = lr.lo; mu <= lr.hi; mu++)
                 for (int p = 0; p < 2; p++) std::swap(e->cur.p[mu][p], e->alt.p[mu][p]);
     }
     v.top_saved = true;
@@ -1174,13 +1199,14 @@ int lgt_ape_vjp_backward(lgt_engine* e, const double* cotangent, double* grad)
             for (int n = n0; n < n1; n++) {
                 const RawPlanes& in = (n == n0) ? v.ckP[s] : v.segUP[n - n0 - 1];
                 TRY(run(e, LGT_K_APE_SAVE, "ape recompute", [&] {
-                    return DISPATCH(e, ape_save(e->stream, ugeo(e, LGT_K_APE_SAVE, 0), in, v.segUP[n - n0], v.segMN[n - n0], sp));
+                    return DISPATCH(e, ape_save(e->stream, ugeo(e, LGT_K_APE_SAVE, 0), in, v.segUP[n - n0], v.segMN[n - n0],
+                                                v.mode, sp));
                 }));
             }
         }
         if (s == v.nck - 1) {
             TRY(run(e, LGT_K_VJP, "adjoint tangent", [&] {
-                return DISPATCH(e, vjp_tangent(e->stream, e->geo, e->cur, v.gobjP, v.segMN[n1 - 1 - n0], v.gP[h]));
+                return DISPATCH(e, vjp_tangent(e->stream, e->geo, e->cur, v.gobjP, v.segMN[n1 - 1 - n0], v.gP[h], v.mode));
             }));
         }
         for (int n = n1 - 1; n >= n0; n--) {
@@ -1188,15 +1214,18 @@ int lgt_ape_vjp_backward(lgt_engine* e, const double* cotangent, double* grad)
             // past the bottom of a segment (s >= 1) the |M| kept by the forward
             const RawNorms* mprev = (n > n0) ? &v.segMN[n - 1 - n0] : (s > 0 ? &v.ckMN[s] : nullptr);
             TRY(run(e, LGT_K_VJP, "adjoint step", [&] {
-                return DISPATCH(e, vjp_step(e->stream, ugeo(e, LGT_K_VJP, 0), Z, v.gP[h], mprev, v.gP[1 - h], sp));
+                return DISPATCH(e, vjp_step(e->stream, ugeo(e, LGT_K_VJP, 0), Z, v.gP[h], mprev, v.gP[1 - h], v.mode, sp));
             }));
             h = 1 - h;
         }
     }
-    // spatial links: the adjoint's result; temporal links: the cotangent itself
+    // the carried links: the adjoint's result; the temporal links of spatial
+    // smearing (in no staple): the cotangent itself
     RawPlanes out = v.gP[h];
-    out.p[0][0] = v.gobjP.p[0][0];
-    out.p[0][1] = v.gobjP.p[0][1];
+    if (v.mode == 0) {
+        out.p[0][0] = v.gobjP.p[0][0];
+        out.p[0][1] = v.gobjP.p[0][1];
+    }
     if (grad) return field_to_host(e, out, grad);
     HIP_TRY(hipStreamSynchronize(e->stream), "hipStreamSynchronize");
     return LGT_OK;

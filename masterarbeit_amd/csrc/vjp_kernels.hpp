@@ -1,21 +1,26 @@
-// vjp_kernels.hpp -- adjoint (vector-Jacobian product) of N-step spatial APE
-// smearing and the gradient of the Wilson-loop objective
+// vjp_kernels.hpp -- adjoint (vector-Jacobian product) of N-step APE
+// smearing (spatial, all directions, temporal) and the gradient of the
+// Wilson-loop objective
 //   L = sum_x sum_{d=1..3} 2 Re[W_{R x T}(x; t, d)]_0 .
 // No reference counterpart.  Gradients are Euclidean in the 4 real link
 // components, <a,b> = a0 b0 + a.b = Re tr(a b^+)/2; right (left)
 // multiplication by q has adjoint right (left) multiplication by q^+.
 //
-// One forward step (Jacobi, spatial links a = 1..3):
-//   M_a(y) = Z_a(y) + sp S_a(y),   U'_a(y) = M_a(y) / |M_a(y)|
+// One forward step (Jacobi) smears the links a of the mode's set D:
+//   M_a(y) = Z_a(y) + sp S_a(y),   U'_a(y) = M_a(y) / |M_a(y)|,
+// the staple S_a summing over the mode's staple directions K(a) (ApeDirs).
 // Backward of that step for a cotangent G' of U':
-//   H_a(y)  = (G'_a(y) - <G'_a(y), U'_a(y)> U'_a(y)) / |M_a(y)|      (tangent)
-//   G_a(y)  = H_a(y) + sp * sum_{b != a} (t1 + ... + t6)            (gather)
-// with, for every spatial b != a, the six places where Z_a(y) enters a staple:
+//   H_a(y)  = (G'_a(y) - <G'_a(y), U'_a(y)> U'_a(y)) / |M_a(y)|,  a in D
+//   H_a(y)  = G'_a(y),                                           a not in D
+//   G_a(y)  = H_a(y) + sp * sum_{b != a} (terms below)            (gather)
+// where Z_a(y) enters a staple: as a side link of a staple of U'_b (b in D,
+// a in K(b)): t1, t3, t4, t6; as the middle link of a staple of U'_a at a
+// neighbour (a in D, b in K(a)): t2, t5:
 //   t1 = H_b(y) Z_a(y+b) Z_b(y+a)^+          t4 = Z_b(y) Z_a(y+b) H_b(y+a)^+
 //   t2 = Z_b(y-b)^+ H_a(y-b) Z_b(y-b+a)      t5 = Z_b(y) H_a(y+b) Z_b(y+a)^+
 //   t3 = H_b(y-b)^+ Z_a(y-b) Z_b(y-b+a)      t6 = Z_b(y-b)^+ Z_a(y-b) H_b(y-b+a)
-// Temporal links are not used by spatial staples: their cotangent passes
-// through every step unchanged.
+// Links in no staple of the mode (the temporal ones of spatial APE) keep
+// their cotangent through every step.
 #pragma once
 
 #include "kernels.hpp"
@@ -23,103 +28,142 @@
 namespace lgt {
 
 
-// H = (G - <G,U> U) / |M| on the spatial links
-template <typename T>
+// Directions of the APE modes (LGT_Code/modules/obs/smearing.c:74-168): the
+// smeared links j in [LO, HI]; the staple directions k of j; the directions
+// whose cotangent the adjoint carries, [CLO, 3] (all that a staple touches).
+template <int MODE> struct ApeDirs {
+    static constexpr int LO = (MODE == 0) ? 1 : 0, HI = (MODE == 2) ? 0 : 3, CLO = (MODE == 0) ? 1 : 0;
+    static constexpr bool smeared(int j) { return j >= LO && j <= HI; }
+    static constexpr bool staple(int j, int k) { return smeared(j) && k != j && (MODE == 1 || k != 0); }
+};
+
+// H = (G - <G,U> U) / |M| on the smeared links, H = G on the other carried ones
+// grid: x over cb sites, y = (direction - CLO) * 2 + parity
+template <typename T, int MODE>
 __global__ void __launch_bounds__(BLOCK) k_vjp_tangent(Planes<T> U, Planes<T> G, NormPlanes<T> Mn, Planes<T> H, Geo g)
 {
 #pragma clang fp contract(fast)
+    typedef ApeDirs<MODE> D;
     uint32_t i = blockIdx.x * BLOCK + threadIdx.x;
     if (i >= g.vh) return;
     int p = blockIdx.y & 1;
-    int mu = 1 + (blockIdx.y >> 1);
-    Q<T> u = qload<T>(U.p[mu][p], i), gr = qload<T>(G.p[mu][p], i);
-    T c = qdot(gr, u), m = Mn.p[mu][p][i];
-    Q<T> h{(gr.c0 - c * u.c0) / m, (gr.c1 - c * u.c1) / m, (gr.c2 - c * u.c2) / m, (gr.c3 - c * u.c3) / m};
-    qstore<T>(H.p[mu][p], i, h);
+    int mu = D::CLO + (int)(blockIdx.y >> 1);
+    Q<T> gr = qload<T>(G.p[mu][p], i);
+    if (D::smeared(mu)) {
+        Q<T> u = qload<T>(U.p[mu][p], i);
+        T c = qdot(gr, u), m = Mn.p[mu][p][i];
+        gr = Q<T>{(gr.c0 - c * u.c0) / m, (gr.c1 - c * u.c1) / m, (gr.c2 - c * u.c2) / m, (gr.c3 - c * u.c3) / m};
+    }
+    qstore<T>(H.p[mu][p], i, gr);
 }
 
 // One backward step.  Z = input field of the step, Hin = tangent cotangent of
-// its output (H above).  Writes G = dL/dZ for the spatial links, or, when
+// its output (H above).  Writes G = dL/dZ for the carried links, or, when
 // Mprev is given (|M| of the step before), directly the tangent cotangent of
-// that earlier step: (G - <G,Z> Z)/Mprev  (Z is that step's output).
-// One thread handles the three spatial links of its site and loads each of
-// the 48 (Z, H) links they need once, in batches: the six at y; for every
-// pair {a,b} the four at y+a and y+b, which close the forward staples of both
-// G_a and G_b; for every b the ten around y-b, which close the backward
-// staples of the two G_a with a != b.  A batch's loads are issued together
-// before its products (its latency is exposed once) and only one batch is
-// live at a time (FP64: 166 VGPRs, three waves per SIMD).
+// that earlier step: (G - <G,Z> Z)/Mprev on the smeared links (Z is that
+// step's output), G on the others.
+// One thread handles the carried links of its site and loads each (Z, H) link
+// they need once (spatial: 48), in batches: the links at y; for every pair
+// {a,b} the four at y+a and y+b, which close the forward staples of both G_a
+// and G_b; for every b those around y-b, which close the backward staples of
+// the G_a, a != b.  A batch's loads are issued together before its products
+// (its latency is exposed once) and only one batch is live at a time
+// (spatial FP64: 166 VGPRs, three waves per SIMD).  A term the mode does not
+// have is neither loaded nor computed; the spatial mode's arithmetic is the
+// same operation for operation as with every term present.
 // grid: x over (tile, parity) pairs
-template <typename T>
+template <typename T, int MODE>
 __global__ void __launch_bounds__(BLOCK) k_vjp_step(Planes<T> Z, Planes<T> Hin, NormPlanes<T> Mprev, int has_prev,
                                                     Planes<T> Out, Geo g, T sp)
 {
 #pragma clang fp contract(fast)
+    typedef ApeDirs<MODE> D;
+    constexpr int C0 = D::CLO;
     int p;
     uint32_t i = tile_base_pair(g, p) + threadIdx.x;
     if (i >= g.vh) return;
     const Site y = cb_site(g, i, p);
     Q<T> Zy[4], Hy[4], acc[4];
 #pragma unroll
-    for (int a = 1; a < 4; a++) {
+    for (int a = C0; a < 4; a++) {
         Zy[a] = link(Z, g, a, y);
         Hy[a] = link(Hin, g, a, y);
         acc[a] = qzero<T>();
     }
-    // batches 1-3: for each pair {a,b} the forward staples through y+a, y+b
+    // forward staples: for each pair {a,b} the links at y+a and y+b
 #pragma unroll
-    for (int a = 1; a < 4; a++)
+    for (int a = C0; a < 4; a++)
 #pragma unroll
         for (int b = a + 1; b < 4; b++) {
+            const bool side_ab = D::staple(b, a), mid_ab = D::staple(a, b);   // terms of G_a
+            const bool side_ba = D::staple(a, b), mid_ba = D::staple(b, a);   // terms of G_b
+            if (!(side_ab || mid_ab || side_ba || mid_ba)) continue;
 #pragma unroll
-            for (int c = 1; c < 4; c++) pin(acc[c]);
+            for (int c = C0; c < 4; c++) pin(acc[c]);
             asm volatile("" ::: "memory");
             const Site ya = shift(g, y, a, 1), yb = shift(g, y, b, 1);
             const Q<T> Zab = link(Z, g, a, yb), Hab = link(Hin, g, a, yb);   // Z_a(y+b), H_a(y+b)
             const Q<T> Zba = link(Z, g, b, ya), Hba = link(Hin, g, b, ya);   // Z_b(y+a), H_b(y+a)
             asm volatile("" ::: "memory");
             // t1 + t5 + t4 of G_a = (H_b(y) Z_a(y+b) + Z_b(y) H_a(y+b)) Z_b(y+a)^+ + Z_b(y) Z_a(y+b) H_b(y+a)^+
-            Q<T> up = qadd(qmul(Hy[b], Zab), qmul(Zy[b], Hab));
-            acc[a] = qadd(acc[a], qadd(qmul_dag(up, Zba), qmul_dag(qmul(Zy[b], Zab), Hba)));
+            if (side_ab || mid_ab) {
+                const Q<T> up = (side_ab && mid_ab) ? qadd(qmul(Hy[b], Zab), qmul(Zy[b], Hab))
+                                                   : (side_ab ? qmul(Hy[b], Zab) : qmul(Zy[b], Hab));
+                Q<T> t = qmul_dag(up, Zba);
+                if (side_ab) t = qadd(t, qmul_dag(qmul(Zy[b], Zab), Hba));
+                acc[a] = qadd(acc[a], t);
+            }
             // the same for G_b, a and b exchanged
-            up = qadd(qmul(Hy[a], Zba), qmul(Zy[a], Hba));
-            acc[b] = qadd(acc[b], qadd(qmul_dag(up, Zab), qmul_dag(qmul(Zy[a], Zba), Hab)));
+            if (side_ba || mid_ba) {
+                const Q<T> up = (side_ba && mid_ba) ? qadd(qmul(Hy[a], Zba), qmul(Zy[a], Hba))
+                                                   : (side_ba ? qmul(Hy[a], Zba) : qmul(Zy[a], Hba));
+                Q<T> t = qmul_dag(up, Zab);
+                if (side_ba) t = qadd(t, qmul_dag(qmul(Zy[a], Zba), Hab));
+                acc[b] = qadd(acc[b], t);
+            }
         }
-    // batches 4-6: for each b the backward staples through y-b
+    // backward staples: for each b the links around y-b
 #pragma unroll
-    for (int b = 1; b < 4; b++) {
+    for (int b = C0; b < 4; b++) {
+        bool any = false;
+#pragma unroll
+        for (int a = C0; a < 4; a++) any = any || (a != b && (D::staple(b, a) || D::staple(a, b)));
+        if (!any) continue;
         // the previous batch's products are complete before this batch's loads
         // are issued (its links are dead by then: one batch live at a time)
 #pragma unroll
-        for (int a = 1; a < 4; a++) pin(acc[a]);
+        for (int a = C0; a < 4; a++) pin(acc[a]);
         asm volatile("" ::: "memory");
         const Site ym = shift(g, y, b, -1);
         const Q<T> Zb = link(Z, g, b, ym), Hb = link(Hin, g, b, ym);   // Z_b(y-b), H_b(y-b)
-        Q<T> C[4][2], D[4][2];   // C[a] = (Z_a(y-b), H_a(y-b)), D[a] = (Z_b(y-b+a), H_b(y-b+a))
+        Q<T> C[4][2], E[4][2];   // C[a] = (Z_a(y-b), H_a(y-b)), E[a] = (Z_b(y-b+a), H_b(y-b+a))
 #pragma unroll
-        for (int a = 1; a < 4; a++) {
-            if (a == b) continue;
+        for (int a = C0; a < 4; a++) {
+            if (a == b || !(D::staple(b, a) || D::staple(a, b))) continue;
             const Site yma = shift(g, ym, a, 1);
             C[a][0] = link(Z, g, a, ym);
             C[a][1] = link(Hin, g, a, ym);
-            D[a][0] = link(Z, g, b, yma);
-            D[a][1] = link(Hin, g, b, yma);
+            E[a][0] = link(Z, g, b, yma);
+            E[a][1] = link(Hin, g, b, yma);
         }
         asm volatile("" ::: "memory");
         const Q<T> Zbd = qdag(Zb), Hbd = qdag(Hb);
 #pragma unroll
-        for (int a = 1; a < 4; a++) {
-            if (a == b) continue;
+        for (int a = C0; a < 4; a++) {
+            const bool side = D::staple(b, a), mid = D::staple(a, b);
+            if (a == b || !(side || mid)) continue;
             // t2 + t3 + t6 = (Z_b(y-b)^+ H_a(y-b) + H_b(y-b)^+ Z_a(y-b)) Z_b(y-b+a) + Z_b(y-b)^+ Z_a(y-b) H_b(y-b+a)
-            const Q<T> lo = qadd(qmul(Zbd, C[a][1]), qmul(Hbd, C[a][0]));
-            const Q<T> zl = qmul(Zbd, C[a][0]);
-            acc[a] = qadd(acc[a], qadd(qmul(lo, D[a][0]), qmul(zl, D[a][1])));
+            const Q<T> lo = (side && mid) ? qadd(qmul(Zbd, C[a][1]), qmul(Hbd, C[a][0]))
+                                          : (mid ? qmul(Zbd, C[a][1]) : qmul(Hbd, C[a][0]));
+            Q<T> t = qmul(lo, E[a][0]);
+            if (side) t = qadd(t, qmul(qmul(Zbd, C[a][0]), E[a][1]));
+            acc[a] = qadd(acc[a], t);
         }
     }
 #pragma unroll
-    for (int a = 1; a < 4; a++) {
+    for (int a = C0; a < 4; a++) {
         Q<T> G = qadd(Hy[a], qscale(acc[a], sp));
-        if (has_prev) {
+        if (has_prev && D::smeared(a)) {
             const Q<T>& z = Zy[a];
             T c = qdot(G, z), m = Mprev.p[a][p][i];
             G = Q<T>{(G.c0 - c * z.c0) / m, (G.c1 - c * z.c1) / m, (G.c2 - c * z.c2) / m, (G.c3 - c * z.c3) / m};

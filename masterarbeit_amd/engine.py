@@ -54,7 +54,7 @@ SIGNATURES = {
     "lgt_ape": (C.c_int, [C.c_void_p, C.c_int, C.c_int, C.c_double]),
     "lgt_wilson_loops": (C.c_int, [C.c_void_p, C.c_int, _ip, _ip, _dp]),
     "lgt_qtop": (C.c_int, [C.c_void_p, C.c_int, _dp, _dp, _dp]),
-    "lgt_ape_vjp_forward": (C.c_int, [C.c_void_p, C.c_int, C.c_double, C.c_int]),
+    "lgt_ape_vjp_forward": (C.c_int, [C.c_void_p, C.c_int, C.c_int, C.c_double, C.c_int]),
     "lgt_ape_vjp_backward": (C.c_int, [C.c_void_p, _dp, _dp]),
     "lgt_wloop_sum": (C.c_int, [C.c_void_p, C.c_int, C.c_int, _dp]),
     "lgt_wloop_grad": (C.c_int, [C.c_void_p, C.c_int, C.c_int, _dp]),
@@ -250,9 +250,9 @@ class Engine:
         return (q.value, s.value, qx) if density else (q.value, s.value)
 
 
-    # ---- adjoint of spatial APE
-    def ape_vjp_forward(self, nsteps, sp, segment=0):
-        _check(self._lib.lgt_ape_vjp_forward(self._h, nsteps, sp, segment), "lgt_ape_vjp_forward")
+    # ---- adjoint of APE smearing (mode: APE_SPATIAL, APE_ALL, APE_TEMPORAL)
+    def ape_vjp_forward(self, nsteps, sp, segment=0, mode=APE_SPATIAL):
+        _check(self._lib.lgt_ape_vjp_forward(self._h, mode, nsteps, sp, segment), "lgt_ape_vjp_forward")
 
     def ape_vjp_backward(self, cotangent=None, grad=True):
         cot = None if cotangent is None else np.ascontiguousarray(cotangent, dtype=np.float64)

@@ -896,14 +896,21 @@ static void qmul3(const double *a, const double *b, const double *c, int dag, do
     or_qmul(t, Cc, o);
 }
 
-/* one spatial APE step U -> U' storing |M| = |U + sp S| per spatial link */
-static void ape_spatial_step_save(const olat_t *lat, const double *Z, double *Uo, double *Mn, double sp)
+/* smeared directions j and staple directions k of the three modes
+ * (smearing.c:74-168): spatial j = 1..3, k in 1..3 \ {j}; all j = 0..3,
+ * k != j; temporal j = 0, k = 1..3 */
+static int ape_lo(int mode) { return mode == 0 ? 1 : 0; }
+static int ape_hi(int mode) { return mode == 2 ? 0 : 3; }
+static int ape_k_ok(int mode, int j, int k) { return k != j && (mode == 1 || k != 0); }
+
+/* one APE step of the given mode that also keeps |M| of every smeared link */
+static void ape_step_save(const olat_t *lat, int mode, const double *Z, double *Uo, double *Mn, double sp)
 {
     memcpy(Uo, Z, sizeof(double) * 16 * (size_t)lat->vol);
     for (int64_t n = 0; n < lat->vol; n++)
-        for (int j = 1; j < 4; j++) {
+        for (int j = ape_lo(mode); j <= ape_hi(mode); j++) {
             double S[4], M[4];
-            or_ape_staple(lat, Z, n, j, 0, S);
+            or_ape_staple(lat, Z, n, j, (mode == 1) ? j : 0, S);
             for (int c = 0; c < 4; c++) M[c] = CLINK(Z, n, j)[c] + sp * S[c];
             double m = sqrt(qdot4(M, M));
             Mn[4 * n + j] = m;
@@ -911,15 +918,17 @@ static void ape_spatial_step_save(const olat_t *lat, const double *Z, double *Uo
         }
 }
 
-/* backward of one step: Gout = dL/dU' -> Gin = dL/dZ */
-static void ape_spatial_step_backward(const olat_t *lat, const double *Z, const double *Uo, const double *Mn,
-                                      double sp, const double *Gout, double *Gin)
+/* backward of one step: Gout = dL/dU' -> Gin = dL/dZ.  Links that the mode
+ * does not smear pass their cotangent through and collect what they
+ * contribute to the staples of the smeared ones. */
+static void ape_step_backward(const olat_t *lat, int mode, const double *Z, const double *Uo, const double *Mn,
+                              double sp, const double *Gout, double *Gin)
 {
     const int64_t V = lat->vol;
     double *H = (double *)calloc((size_t)16 * V, sizeof(double));
-    memcpy(Gin, Gout, sizeof(double) * 16 * (size_t)V);     /* temporal links pass through */
+    memcpy(Gin, Gout, sizeof(double) * 16 * (size_t)V);
     for (int64_t n = 0; n < V; n++)
-        for (int j = 1; j < 4; j++) {
+        for (int j = ape_lo(mode); j <= ape_hi(mode); j++) {
             double g[4];
             memcpy(g, CLINK(Gout, n, j), sizeof(g));
             qsub_scaled(g, CLINK(Uo, n, j), qdot4(g, CLINK(Uo, n, j)));
@@ -930,11 +939,11 @@ static void ape_spatial_step_backward(const olat_t *lat, const double *Z, const 
             }
         }
     for (int64_t x = 0; x < V; x++)
-        for (int j = 1; j < 4; j++) {
+        for (int j = ape_lo(mode); j <= ape_hi(mode); j++) {
             double h[4], t[4];
             for (int c = 0; c < 4; c++) h[c] = sp * H[16 * x + 4 * j + c];
-            for (int k = 1; k < 4; k++) {
-                if (k == j) continue;
+            for (int k = 0; k < 4; k++) {
+                if (!ape_k_ok(mode, j, k)) continue;
                 int64_t xk = NB(lat, x, k), xj = NB(lat, x, j);
                 /* forward term Z_k(x) Z_j(x+k) Z_k(x+j)^+ */
                 qmul3(h, CLINK(Z, xj, k), CLINK(Z, xk, j), 4, t);          /* h Z_k(x+j) Z_j(x+k)^+ */
@@ -1008,7 +1017,7 @@ static void wloop_grad(const olat_t *lat, const double *U, int R, int T, double 
     free(site); free(dir); free(dag); free(pre); free(suf);
 }
 
-double or_ape_vjp(const olat_t *lat, const double *U, int nsteps, double sp, int R, int T, double *G)
+double or_ape_vjp(const olat_t *lat, const double *U, int mode, int nsteps, double sp, int R, int T, double *G)
 {
     size_t fb = sizeof(double) * 16 * (size_t)lat->vol;
     double *fields = (double *)malloc(fb * (size_t)(nsteps + 1));
@@ -1016,14 +1025,14 @@ double or_ape_vjp(const olat_t *lat, const double *U, int nsteps, double sp, int
     double *g2 = (double *)malloc(fb);
     memcpy(fields, U, fb);
     for (int s = 0; s < nsteps; s++)
-        ape_spatial_step_save(lat, fields + 16 * lat->vol * s, fields + 16 * lat->vol * (s + 1),
-                              mn + 4 * lat->vol * s, sp);
+        ape_step_save(lat, mode, fields + 16 * lat->vol * s, fields + 16 * lat->vol * (s + 1),
+                      mn + 4 * lat->vol * s, sp);
     const double *Uf = fields + 16 * lat->vol * nsteps;
     double L = or_wloop_objective(lat, Uf, R, T);
     wloop_grad(lat, Uf, R, T, G);
     for (int s = nsteps - 1; s >= 0; s--) {
-        ape_spatial_step_backward(lat, fields + 16 * lat->vol * s, fields + 16 * lat->vol * (s + 1),
-                                  mn + 4 * lat->vol * s, sp, G, g2);
+        ape_step_backward(lat, mode, fields + 16 * lat->vol * s, fields + 16 * lat->vol * (s + 1),
+                          mn + 4 * lat->vol * s, sp, G, g2);
         memcpy(G, g2, fb);
     }
     free(fields); free(mn); free(g2);

@@ -141,12 +141,13 @@ int  or_wilson_loops(const olat_t *lat, const double *U, int nloop, const int *R
 double or_qtop_improved(const olat_t *lat, const double *U, int seq, double *qx, double *abs_sum);
 double or_qtop_clover(const olat_t *lat, const double *U, int seq, double *qx, double *abs_sum);
 
-/* ---- adjoint of spatial APE smearing ----
- * L = sum_x sum_{d=1..3} 2*Re[ W_{R x T}(x; 0, d) ]_0 on the smeared field.
- * Writes dL/dU (4 real components per link, Euclidean gradient) for the thin
- * field into G (same layout as U); returns L. */
+/* ---- adjoint of APE smearing ----
+ * L = sum_x sum_{d=1..3} 2*Re[ W_{R x T}(x; 0, d) ]_0 on the field smeared
+ * by nsteps APE steps of the given mode (0 spatial, 1 all, 2 temporal, as
+ * or_ape).  Writes dL/dU (4 real components per link, Euclidean gradient)
+ * for the thin field into G (same layout as U); returns L. */
 double or_wloop_objective(const olat_t *lat, const double *U, int R, int T);
-double or_ape_vjp(const olat_t *lat, const double *U, int nsteps, double sp, int R, int T,
+double or_ape_vjp(const olat_t *lat, const double *U, int mode, int nsteps, double sp, int R, int T,
                   double *G);
 
 #ifdef __cplusplus

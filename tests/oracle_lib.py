@@ -78,7 +78,7 @@ class Oracle:
                 "or_qtop_improved": (C.c_double, [P, _dp, C.c_int, _dp, _dp]),
                 "or_qtop_clover": (C.c_double, [P, _dp, C.c_int, _dp, _dp]),
                 "or_wloop_objective": (C.c_double, [P, _dp, C.c_int, C.c_int]),
-                "or_ape_vjp": (C.c_double, [P, _dp, C.c_int, C.c_double, C.c_int, C.c_int, _dp]),
+                "or_ape_vjp": (C.c_double, [P, _dp, C.c_int, C.c_int, C.c_double, C.c_int, C.c_int, _dp]),
                 "or_metropolis_batch": (C.c_int, [C.c_int64, _dp, _dp, C.c_double, C.c_double, _dp, _dp, _ip]),
                 "or_metro_subsweep_philox": (C.c_int64, [P, _dp, C.c_int, C.c_int, C.c_double, C.c_double,
                                                           C.c_uint64, C.c_uint32]),
@@ -188,9 +188,9 @@ class Oracle:
     def wloop_objective(self, U, R, T):
         return self.lib().or_wloop_objective(C.byref(self.lat), _ptr(U), R, T)
 
-    def ape_vjp(self, U, nsteps, sp, R, T):
+    def ape_vjp(self, U, nsteps, sp, R, T, mode=0):
         G = self.field()
-        L = self.lib().or_ape_vjp(C.byref(self.lat), _ptr(U), nsteps, sp, R, T, _ptr(G))
+        L = self.lib().or_ape_vjp(C.byref(self.lat), _ptr(U), mode, nsteps, sp, R, T, _ptr(G))
         return L, G
 
 

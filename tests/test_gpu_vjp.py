@@ -1,5 +1,5 @@
-"""Adjoint of N-step spatial APE smearing with the Wilson-loop objective
-L = sum_x sum_d 2 Re[W_{RxT}]_0, through the C ABI.
+"""Adjoint of N-step APE smearing (spatial, all directions, temporal) with
+the Wilson-loop objective L = sum_x sum_d 2 Re[W_{RxT}]_0, through the C ABI.
 
 Checked against the FP64 CPU chain rule (oracle/su2_oracle.c, or_ape_vjp,
 which scatters each staple term's factor gradients while the engine gathers
@@ -48,74 +48,103 @@ def test_wloop_sum_and_gradient(lgt, L):
     np.testing.assert_allclose(G, Gref, rtol=0, atol=1e-12 * np.abs(Gref).max())
 
 
+MODES = [(0, 0.25), (1, 1.0 / 6.0), (2, 0.25)]     # (mode, sp): spatial, all directions, temporal
+
+
+@pytest.mark.parametrize("mode,sp", MODES)
 @pytest.mark.parametrize("L,nsteps,segment", [((6, 4, 4, 8), 7, 0), ((6, 4, 4, 8), 7, 3), ((8, 8, 8, 8), 5, 5),
                                               ((8, 8, 8, 8), 9, 2)])
-def test_vjp_matches_cpu_chain_rule(lgt, L, nsteps, segment):
+def test_vjp_matches_cpu_chain_rule(lgt, L, nsteps, segment, mode, sp):
     o, U = thermal(L)
-    R, T, sp = 2, 2, 0.25
-    Lref, Gref = o.ape_vjp(U, nsteps, sp, R, T)
+    R, T = 2, 2
+    Lref, Gref = o.ape_vjp(U, nsteps, sp, R, T, mode)
     with lgt.Engine(L, 0, 1) as e:
         e.upload(U)
-        e.ape_vjp_forward(nsteps, sp, segment)
+        e.ape_vjp_forward(nsteps, sp, segment, mode=mode)
         hi, lo = e.wloop_sum(R, T)
         e.wloop_grad(R, T, host=False)
         G = e.ape_vjp_backward()
         V = e.download()
     W = U.copy()
-    o.ape(W, 0, nsteps, sp)
+    o.ape(W, mode, nsteps, sp)
     np.testing.assert_array_equal(V, W)              # the forward pass is the plain smearing, bit for bit
     assert hi + lo == pytest.approx(Lref, rel=1e-13)
     scale = np.abs(Gref).max()
     err = np.abs(G - Gref).max() / scale
-    print(f"VJP L={L} N={nsteps} seg={segment}: max rel err {err:.2e}")
+    print(f"VJP mode={mode} L={L} N={nsteps} seg={segment}: max rel err {err:.2e}")
     assert err < 1e-11
 
 
-def test_vjp_explicit_cotangent(lgt):
+def test_vjp_modes_share_an_engine(lgt):
+    """Alternating modes on one engine (each re-lays out the buffers) gives
+    the same gradients as fresh engines."""
+    L = (6, 4, 4, 8)
+    o, U = thermal(L)
+    res = {}
+    with lgt.Engine(L, 0, 1) as e:
+        for mode, sp in MODES + MODES[:1]:
+            e.upload(U)
+            e.ape_vjp_forward(5, sp, mode=mode)
+            e.wloop_grad(2, 2, host=False)
+            res.setdefault(mode, []).append(e.ape_vjp_backward())
+    for mode, sp in MODES:
+        with lgt.Engine(L, 0, 1) as e:
+            e.upload(U)
+            e.ape_vjp_forward(5, sp, mode=mode)
+            e.wloop_grad(2, 2, host=False)
+            G = e.ape_vjp_backward()
+        for Gm in res[mode]:
+            np.testing.assert_array_equal(Gm, G)
+
+
+@pytest.mark.parametrize("mode,sp", MODES)
+def test_vjp_explicit_cotangent(lgt, mode, sp):
     L = (6, 4, 4, 8)
     o, U = thermal(L)
     rng = np.random.default_rng(4)
     cot = rng.normal(size=(o.vol, 4, 4))
     with lgt.Engine(L, 0, 1) as e:
         e.upload(U)
-        e.ape_vjp_forward(4, 0.25)
+        e.ape_vjp_forward(4, sp, mode=mode)
         G1 = e.ape_vjp_backward(cot)
         G2 = e.ape_vjp_backward(cot)                 # repeatable, the forward state is kept
     np.testing.assert_array_equal(G1, G2)
-    np.testing.assert_array_equal(G1[:, 0], cot[:, 0])   # temporal links pass through
+    if mode == 0:
+        np.testing.assert_array_equal(G1[:, 0], cot[:, 0])   # temporal links pass through
     # linearity in the cotangent
     with lgt.Engine(L, 0, 1) as e:
         e.upload(U)
-        e.ape_vjp_forward(4, 0.25)
+        e.ape_vjp_forward(4, sp, mode=mode)
         G3 = e.ape_vjp_backward(2.5 * cot)
     np.testing.assert_allclose(G3, 2.5 * G1, rtol=1e-13, atol=1e-13)
 
 
+@pytest.mark.parametrize("mode,sp", MODES)
 @pytest.mark.parametrize("prec,rtol,eps", [(0, 2e-6, 1e-4), (1, 2e-2, 1e-2)])
-def test_vjp_matches_gpu_finite_differences(lgt, prec, rtol, eps):
+def test_vjp_matches_gpu_finite_differences(lgt, prec, rtol, eps, mode, sp):
     L = (8, 8, 8, 8)
     o, U = thermal(L)
-    R, T, sp, nsteps = 3, 3, 0.25, 12
+    R, T, nsteps = 3, 3, 12
     with lgt.Engine(L, prec, 1) as e:
         e.upload(U)
-        e.ape_vjp_forward(nsteps, sp)
+        e.ape_vjp_forward(nsteps, sp, mode=mode)
         e.wloop_grad(R, T, host=False)
         G = e.ape_vjp_backward()
         rng = np.random.default_rng(9)
-        for _ in range(6):
-            n, mu = int(rng.integers(o.vol)), int(rng.integers(4))
+        for i in range(8):
+            n, mu = int(rng.integers(o.vol)), i % 4
             X = np.concatenate([[0.0], rng.normal(size=3)])
             vals = []
             for sgn in (1, -1):
                 V = U.copy()
                 V[n, mu] = F.qmul(su2_exp(X, sgn * eps), U[n, mu])
                 e.upload(V)
-                e.ape(0, nsteps, sp)
+                e.ape(mode, nsteps, sp)
                 hi, lo = e.wloop_sum(R, T)
                 vals.append((hi, lo))
             fd = ((vals[0][0] - vals[1][0]) + (vals[0][1] - vals[1][1])) / (2 * eps)
             an = float(np.dot(G[n, mu], F.qmul(X, U[n, mu])))
-            assert an == pytest.approx(fd, rel=rtol, abs=1e-7 if prec == 0 else 1e-3), (n, mu, an, fd)
+            assert an == pytest.approx(fd, rel=rtol, abs=1e-7 if prec == 0 else 1e-3), (mode, n, mu, an, fd)
 
 
 def test_vjp_400_steps_48_finite_differences(lgt):
