@@ -2,7 +2,7 @@
 """Physics studies on the GPU engine: thermalisation, FP32-storage precision,
 topology and a 96^4 production chain.
 
-    python -m masterarbeit_amd.studies c1|c2|c3|c5|all [--quick]
+    python -m masterarbeit_amd.studies c1|c2|c3|c4|c5|c6|all [--quick]
 
 Each study prints one JSON line on stdout (progress on stderr).  The field is
 resident on the device throughout; host transfers happen only where a
@@ -19,9 +19,16 @@ and FP64 storage) or restored after a destructive measurement.
   c3  48^4, seed 7: 500 sweeps (FP64); 4D APE (sp 1/6) for 15 and 30 steps and
       the improved charge Q, on the same configuration in FP64 and FP32
       storage; |dQ|.
+  c4  48^4, FP64, seed 11, 100 sweeps: the adjoint of 400 spatial APE steps
+      (sqrt(N) = 20 checkpoints) for L = sum of the 5x5 loops, against central
+      differences (eps = 1e-4, loop sums in double-double) along 16 random
+      su(2) directions on random links.
   c5  96^4, FP32 storage, hot start (seed 3): 200 sweeps; 400 spatial APE
       steps and W at R = T in {5, 7, 9}; 30 steps of 4D APE and Q; device
       memory against the 288 GB of one MI355X.
+  c6  48^4: <P> of FP32-storage and FP64 chains from 8 seeds (7..14), 200
+      thermalisation + 1500 measured sweeps each; the differences against
+      their autocorrelation-corrected errors.
 """
 import argparse
 import json
@@ -186,6 +193,93 @@ def c3(quick=False):
     return out
 
 
+def su2_exp(X, eps):
+    """exp(eps X) for a pure-imaginary quaternion X = (0, x)."""
+    n = np.linalg.norm(X[1:])
+    return np.concatenate([[np.cos(eps * n)], np.sin(eps * n) * X[1:] / n])
+
+
+def qmul(a, b):
+    """Quaternion product a b (the engine's link algebra)."""
+    return np.array([a[0] * b[0] - a[1] * b[1] - a[2] * b[2] - a[3] * b[3],
+                     a[0] * b[1] + a[1] * b[0] - a[2] * b[3] + a[3] * b[2],
+                     a[0] * b[2] + a[2] * b[0] - a[3] * b[1] + a[1] * b[3],
+                     a[0] * b[3] + a[3] * b[0] - a[1] * b[2] + a[2] * b[1]])
+
+
+def c4(quick=False):
+    L = (48, 48, 48, 48)
+    R = T = 5
+    N = 100 if quick else 400
+    ndir = 4 if quick else 16
+    t0 = time.perf_counter()
+    with E.Engine(L, E.FP64, 11) as e:
+        e.init_field(1)
+        e.update(100, 1, BETA)
+        U = e.download()
+        t1 = time.perf_counter()
+        e.ape_vjp_forward(N, SP_SPATIAL)
+        e.sync()
+        t_fwd = time.perf_counter() - t1
+        e.wloop_grad(R, T, host=False)
+        t1 = time.perf_counter()
+        G = e.ape_vjp_backward()
+        t_bwd = time.perf_counter() - t1
+        rng = np.random.default_rng(11)
+        dev = []
+        for _ in range(ndir):
+            n, mu = int(rng.integers(U.shape[0])), int(rng.integers(4))
+            X = np.concatenate([[0.0], rng.normal(size=3)])
+            vals = []
+            for sgn in (1, -1):
+                old = U[n, mu].copy()
+                U[n, mu] = qmul(su2_exp(X, sgn * 1e-4), old)
+                e.upload(U)
+                U[n, mu] = old
+                e.ape(E.APE_SPATIAL, N, SP_SPATIAL)
+                vals.append(e.wloop_sum(R, T))
+            fd = ((vals[0][0] - vals[1][0]) + (vals[0][1] - vals[1][1])) / 2e-4
+            an = float(np.dot(G[n, mu], qmul(X, U[n, mu])))
+            dev.append(abs(an - fd) / max(abs(fd), 1e-30))
+            log(f"[c4] link ({n},{mu}): adjoint {an:.10g} differences {fd:.10g}")
+    return {"study": "c4", "lattice": L, "beta": BETA, "precision": "fp64", "ape_steps": N, "sp": SP_SPATIAL,
+            "loop": [R, T], "directions": ndir, "worst_relative_deviation": float(max(dev)),
+            "median_relative_deviation": float(np.median(dev)), "forward_s": t_fwd, "backward_s": t_bwd,
+            "seconds": time.perf_counter() - t0}
+
+
+def c6(quick=False):
+    L = (48, 48, 48, 48)
+    seeds = (7, 8) if quick else tuple(range(7, 15))
+    nmeas = 300 if quick else 1500
+    t0 = time.perf_counter()
+    rows = []
+    for seed in seeds:
+        r = {"seed": seed}
+        for prec, name in ((E.FP32, "fp32"), (E.FP64, "fp64")):
+            with E.Engine(L, prec, seed) as e:
+                e.init_field(1)
+                e.update(200, 1, BETA)
+                p = []
+                for _ in range(nmeas):
+                    e.update(1, 1, BETA)
+                    p.append(e.plaquette())
+            p = np.array(p)
+            tau, err = tau_int(p)
+            r[name] = {"mean": float(p.mean()), "err": err, "tau_int": tau}
+        r["diff"] = r["fp32"]["mean"] - r["fp64"]["mean"]
+        r["diff_over_sigma"] = r["diff"] / float(np.hypot(r["fp32"]["err"], r["fp64"]["err"]))
+        log(f"[c6] seed {seed}: dP = {r['diff']:.3g} = {r['diff_over_sigma']:+.2f} sigma")
+        rows.append(r)
+    d = np.array([r["diff"] for r in rows])
+    sig = np.array([np.hypot(r["fp32"]["err"], r["fp64"]["err"]) for r in rows])
+    mean_sig = float(np.sqrt(np.sum(sig ** 2)) / len(d))
+    return {"study": "c6", "lattice": L, "beta": BETA, "thermalisation": 200, "measured_sweeps": nmeas,
+            "per_seed": rows, "mean_diff": float(d.mean()), "mean_diff_err": mean_sig,
+            "mean_diff_over_sigma": float(d.mean()) / mean_sig,
+            "rms_z": float(np.sqrt(np.mean((d / sig) ** 2))), "seconds": time.perf_counter() - t0}
+
+
 def c5(quick=False):
     L = (96, 96, 96, 96)
     nsweeps = 20 if quick else 200
@@ -229,11 +323,11 @@ def c5(quick=False):
 
 def main():
     ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
-    ap.add_argument("study", choices=["c1", "c2", "c3", "c5", "all"])
+    ap.add_argument("study", choices=["c1", "c2", "c3", "c4", "c5", "c6", "all"])
     ap.add_argument("--quick", action="store_true", help="fewer sweeps (a functional check)")
     a = ap.parse_args()
     E.load_library()
-    todo = ["c1", "c2", "c3", "c5"] if a.study == "all" else [a.study]
+    todo = ["c1", "c2", "c3", "c4", "c5", "c6"] if a.study == "all" else [a.study]
     for s in todo:
         log(f"[{s}] start")
         r = globals()[s](a.quick)

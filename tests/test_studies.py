@@ -65,3 +65,20 @@ def test_study_c1_on_gpu():
     assert r["plaquette_mean_50_100"] == pytest.approx(0.7325, abs=0.002)
     w = r["wloops"]
     assert len(w) == 36 and 0 < w["6,6"] < w["1,1"] < 1
+
+
+def test_study_quaternion_helpers_match_the_link_algebra():
+    """The finite-difference study's own product and su(2) exponential agree
+    with the test suite's (tests/fields.py) on random quaternions."""
+    import fields as F
+    from masterarbeit_amd.studies import qmul, su2_exp
+    rng = np.random.default_rng(3)
+    for _ in range(20):
+        a, b = rng.normal(size=4), rng.normal(size=4)
+        np.testing.assert_allclose(qmul(a, b), F.qmul(a, b), rtol=1e-14, atol=1e-14)
+        X = np.concatenate([[0.0], rng.normal(size=3)])
+        u = su2_exp(X, 0.3)
+        assert np.linalg.norm(u) == pytest.approx(1.0, abs=1e-15)
+        # exp(eps X) = cos(eps |x|) + sin(eps |x|) x/|x|, and d/deps at 0 is X
+        d = (su2_exp(X, 1e-6) - su2_exp(X, -1e-6)) / 2e-6
+        np.testing.assert_allclose(d, X, rtol=1e-8, atol=1e-8)
