@@ -353,7 +353,7 @@ def main():
     ap.add_argument("--beta", type=float, default=3.09)
     ap.add_argument("--n-or", type=int, default=1)
     ap.add_argument("--seed", type=int, default=7)
-    ap.add_argument("--nplaq", type=int, default=50)
+    ap.add_argument("--nplaq", type=int, default=200)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-table", action="store_true",
                     help="only time the reference and the restatement on the CPU (no GPU) and exit")
@@ -414,6 +414,10 @@ def main():
     p32, p64 = np.array(r32["plaq"]), np.array(r64["plaq"])
     sigma = float(np.sqrt(0.5 * (p32.var(ddof=1) + p64.var(ddof=1)))) if len(p32) > 1 else None
     diff = float(p32.mean() - p64.mean()) if len(p32) else None
+    # errors of the two means with the integrated autocorrelation time
+    # (LGT_Code/Autocorrelation/t_int.c), the chains being independent
+    from masterarbeit_amd.studies import tau_int
+    err_mean = float(np.hypot(tau_int(p32)[1], tau_int(p64)[1])) if len(p32) > 2 else None
 
     base = None
     xfer = time_host_transfer(e64)
@@ -473,6 +477,8 @@ def main():
             "sigma_per_config": sigma,
             "diff": diff,
             "within_1sigma": (abs(diff) <= sigma) if sigma else None,
+            "sigma_of_diff_of_means": err_mean,
+            "diff_over_sigma_of_means": (diff / err_mean) if err_mean else None,
             "n": len(p32),
         },
         "ape_spatial": {"fp32": ape32, "fp64": ape64},
