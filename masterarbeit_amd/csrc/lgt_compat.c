@@ -108,20 +108,39 @@ static void engine(void)
     atexit(lgt_compat_release);
 }
 
+/* The reference's field table points into one allocation, link (s, d) at
+ * base + 4 s + d (init.c:842-869); such a field is handed to the engine in
+ * place.  Returns its base, or NULL when the table points anywhere else (the
+ * links are then packed through the transfer buffer). */
+static double *in_place(su2mat *u[][4])
+{
+    double *base = (double *)u[0][0];
+    for (int64_t s = 0; s < g_vol; s++)
+        for (int d = 0; d < 4; d++)
+            if ((double *)u[s][d] != base + 16 * s + 4 * d) return NULL;
+    return base;
+}
+
 static void upload(su2mat *u[][4])
 {
     engine();
-    for (int64_t s = 0; s < g_vol; s++)
-        for (int d = 0; d < 4; d++) memcpy(g_buf + 16 * s + 4 * d, u[s][d], 4 * sizeof(double));
-    check(lgt_upload(g_eng, g_buf), "lgt_upload");
+    double *f = in_place(u);
+    if (!f) {
+        for (int64_t s = 0; s < g_vol; s++)
+            for (int d = 0; d < 4; d++) memcpy(g_buf + 16 * s + 4 * d, u[s][d], 4 * sizeof(double));
+        f = g_buf;
+    }
+    check(lgt_upload(g_eng, f), "lgt_upload");
     g_uploads++;
 }
 
 static void download(su2mat *u[][4])
 {
-    check(lgt_download(g_eng, g_buf), "lgt_download");
-    for (int64_t s = 0; s < g_vol; s++)
-        for (int d = 0; d < 4; d++) memcpy(u[s][d], g_buf + 16 * s + 4 * d, 4 * sizeof(double));
+    double *f = in_place(u);
+    check(lgt_download(g_eng, f ? f : g_buf), "lgt_download");
+    if (!f)
+        for (int64_t s = 0; s < g_vol; s++)
+            for (int d = 0; d < 4; d++) memcpy(u[s][d], g_buf + 16 * s + 4 * d, 4 * sizeof(double));
     g_downloads++;
 }
 

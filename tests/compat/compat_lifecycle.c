@@ -8,19 +8,25 @@
  */
 #include <stdio.h>
 #include <stdlib.h>
+#include <string.h>
 
 #include "../../include/lgt_compat.h"
 
 extern long driver_vol;
 static double *g_field = NULL;
 
+/* COMPAT_SITE_ORDER=reversed lays the sites out backwards in the allocation,
+ * so that the pointer table is not the reference's contiguous one */
 void initGaugeField(int start)
 {
     (void)start;
+    const char *order = getenv("COMPAT_SITE_ORDER");
+    const int rev = order && !strcmp(order, "reversed");
     g_field = malloc(sizeof(double) * 16 * (size_t)driver_vol);
     for (long n = 0; n < driver_vol; n++)
         for (int d = 0; d < 4; d++) {
-            pu[n][d] = (su2mat *)(g_field + 16 * n + 4 * d);
+            const long slot = rev ? driver_vol - 1 - n : n;
+            pu[n][d] = (su2mat *)(g_field + 16 * slot + 4 * d);
             pu[n][d]->c0 = 1.0;
             pu[n][d]->c1 = pu[n][d]->c2 = pu[n][d]->c3 = 0.0;
         }

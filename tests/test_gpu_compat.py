@@ -29,11 +29,15 @@ def read_summary(path):
     return out
 
 
-@pytest.mark.parametrize("prec,sync", [("fp64", "eager"), ("fp32", "eager"), ("fp64", "lazy"), ("fp32", "lazy")])
-def test_reference_entry_points_through_compat(tmp_path, prec, sync):
+@pytest.mark.parametrize("prec,sync,order", [("fp64", "eager", "contiguous"), ("fp32", "eager", "contiguous"),
+                                             ("fp64", "lazy", "contiguous"), ("fp32", "lazy", "contiguous"),
+                                             ("fp64", "eager", "reversed"), ("fp64", "lazy", "reversed")])
+def test_reference_entry_points_through_compat(tmp_path, prec, sync, order):
     """Both coherence modes give the same results; the lazy build (the
     program's lifecycle functions wrapped at link time) moves the field only
-    when the program itself reads or writes it."""
+    when the program itself reads or writes it.  A field in the reference's
+    contiguous layout goes to the engine in place, any other pointer table
+    (here: sites reversed in the allocation) through the packing buffer."""
     driver = DRIVER if sync == "eager" else DRIVER_LAZY
     if not os.path.exists(driver):
         pytest.fail("compat driver not built (run __graft_entry__.build())")
@@ -42,7 +46,7 @@ def test_reference_entry_points_through_compat(tmp_path, prec, sync):
     U = o.hot(21)
     o.engine_sweep(U, 3.09, 1, 21, 0)
     U.tofile(tmp_path / "field.bin")
-    env = dict(os.environ, LGT_N_OR="1", LGT_PRECISION=prec)
+    env = dict(os.environ, LGT_N_OR="1", LGT_PRECISION=prec, COMPAT_SITE_ORDER=order)
     r = subprocess.run([driver, *map(str, L), str(tmp_path / "field.bin"), str(tmp_path)], env=env,
                        capture_output=True, text=True, timeout=120)
     assert r.returncode == 0, r.stderr
