@@ -50,7 +50,9 @@ template <typename T> struct Launch {
                                   const RawNorms& mn, const RawPlanes& H, int mode);
     static hipError_t vjp_step(hipStream_t s, const Geo& g, const RawPlanes& Z, const RawPlanes& Hin,
                                const RawNorms* mprev, const RawPlanes& out, int mode, double sp);
-    static hipError_t wloop_grad(hipStream_t s, const Geo& g, const RawPlanes& U, int R, int Tt, const RawPlanes& G);
+    // the straight-line fields of the loop gradient (lines_bytes of them, vjp_kernels.hpp)
+    static hipError_t lines(hipStream_t s, const Geo& g, const RawPlanes& U, void* base, int R, int Tt);
+    static hipError_t wloop_grad(hipStream_t s, const Geo& g, void* lines, int R, int Tt, const RawPlanes& G);
 };
 
 // precision-independent launchers (lgt_f64.hip)

@@ -195,9 +195,18 @@ hipError_t Launch<T>::vjp_step(hipStream_t s, const Geo& g, const RawPlanes& Z, 
 }
 
 template <typename T>
-hipError_t Launch<T>::wloop_grad(hipStream_t s, const Geo& g, const RawPlanes& U, int R, int T_, const RawPlanes& G)
+hipError_t Launch<T>::lines(hipStream_t s, const Geo& g, const RawPlanes& U, void* base, int R, int T_)
 {
-    hipLaunchKernelGGL(k_wloop_grad<T>, dim3(2 * grid_x(g), 4), dim3(BLOCK), 0, s, planes<T>(U), g, R, T_, planes<T>(G));
+    Lines<T> Ls{(typename Store<T>::V*)base, R, T_};
+    hipLaunchKernelGGL(k_lines<T>, dim3(2 * grid_x(g), 4), dim3(BLOCK), 0, s, planes<T>(U), Ls, g);
+    return hipGetLastError();
+}
+
+template <typename T>
+hipError_t Launch<T>::wloop_grad(hipStream_t s, const Geo& g, void* lines, int R, int T_, const RawPlanes& G)
+{
+    Lines<T> Ls{(typename Store<T>::V*)lines, R, T_};
+    hipLaunchKernelGGL(k_wloop_grad<T>, dim3(2 * grid_x(g), 4), dim3(BLOCK), 0, s, Ls, g, R, T_, planes<T>(G));
     return hipGetLastError();
 }
 

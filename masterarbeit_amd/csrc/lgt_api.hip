@@ -75,6 +75,8 @@ struct VjpState {
     RawPlanes fixedP;
     void* gobj = nullptr;
     RawPlanes gobjP;
+    void* lines = nullptr;   // straight-line fields of the loop gradient
+    size_t lines_bytes = 0;
 };
 
 }  // namespace
@@ -438,6 +440,8 @@ void free_vjp(lgt_engine* e, bool keep_cot)
     free_buf(e, v.fixed, dir_bytes(e, DirRange{1, 3}, e->link_bytes));
     if (!keep_cot) {
         free_buf(e, v.gobj, (size_t)4 * e->vol * e->link_bytes);
+        free_buf(e, v.lines, v.lines_bytes);
+        v.lines_bytes = 0;
         v.cot = false;
     }
     v.fwd = false;
@@ -1166,8 +1170,12 @@ int lgt_wloop_grad(lgt_engine* e, int R, int T, double* grad)
     TRY(set_dev(e));
     VjpState& v = e->vjp;
     if (!v.gobj) TRY(alloc_field(e, &v.gobj, &v.gobjP));
+    // T temporal and 3R spatial line fields of VOL links
+    TRY(ensure(e, &v.lines, &v.lines_bytes, (size_t)(T + 3 * R) * e->vol * e->link_bytes, "loop lines"));
+    TRY(run(e, LGT_K_WLOOP_GRAD, "loop lines",
+            [&] { return DISPATCH(e, lines(e->stream, kgeo(e, LGT_K_WLOOP_GRAD, 0), e->cur, v.lines, R, T)); }));
     TRY(run(e, LGT_K_WLOOP_GRAD, "wilson-loop gradient",
-            [&] { return DISPATCH(e, wloop_grad(e->stream, kgeo(e, LGT_K_WLOOP_GRAD, 0), e->cur, R, T, v.gobjP)); }));
+            [&] { return DISPATCH(e, wloop_grad(e->stream, kgeo(e, LGT_K_WLOOP_GRAD, 0), v.lines, R, T, v.gobjP)); }));
     v.cot = true;
     if (grad) return field_to_host(e, v.gobjP, grad);
     HIP_TRY(hipStreamSynchronize(e->stream), "hipStreamSynchronize");

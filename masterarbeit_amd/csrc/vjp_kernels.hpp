@@ -177,77 +177,114 @@ __global__ void __launch_bounds__(BLOCK) k_vjp_step(Planes<T> Z, Planes<T> Hin, 
 // (the loop path: T steps +t, R steps +d, T steps -t, R steps -d from x).
 // For a link A at some position of a loop P = X A Y, d(2 Re[P]_0)/dA is
 // 2 (Y X)^+, and 2 Y X where the loop runs the link backwards; Y X is the
-// open path from the end of the link around the loop to its start.
+// open path from the end of the link around the loop to its start.  That
+// path is five straight lines, read from line fields built beforehand:
+//   line(d, n)(z) = U_d(z) U_d(z+d) ... U_d(z+(n-1)d),
+// d = 0 for n = 1..T, d = 1..3 for n = 1..R, one field of VOL links each, at
+// index (d == 0 ? n - 1 : T + (d-1) R + n - 1) laid out like a link direction
+// (parity planes of VOL/2).
 // ---------------------------------------------------------------------------
+template <typename T> struct Lines {
+    typename Store<T>::V* base;
+    int R, Tl;   // spatial and temporal line lengths
+};
+
+// the n-link line along d from z (n = 0: the identity)
 template <typename T>
-__device__ __forceinline__ void seg(const Planes<T>& U, const Geo& g, Q<T>& acc, Site& s, int dir, int n, bool back)
+__device__ __forceinline__ Q<T> line(const Lines<T>& Ls, const Geo& g, int d, int n, const Site& z)
 {
-    for (int k = 0; k < n; k++) {
-        if (!back) {
-            acc = qmul(acc, link(U, g, dir, s));
-            s = shift(g, s, dir, 1);
-        } else {
-            s = shift(g, s, dir, -1);
-            acc = qmul_dag(acc, link(U, g, dir, s));
-        }
+    if (n == 0) return qunit<T>();
+    const size_t idx = (d == 0) ? (size_t)(n - 1) : (size_t)(Ls.Tl + (d - 1) * Ls.R + n - 1);
+    const typename Store<T>::V* f = Ls.base + idx * 2 * (size_t)g.vh;
+    const typename Store<T>::V v = f[(uint32_t)parity(z) * g.vh + cb_index(g, z)];
+    return Q<T>{v.x, v.y, v.z, v.w};
+}
+
+// shift by any number of steps along mu (periodic)
+__device__ __forceinline__ Site shiftn(const Geo& g, Site s, int mu, int n)
+{
+    n %= g.L[mu];
+    if (n < 0) n += g.L[mu];
+    return shift(g, s, mu, n);
+}
+
+// all lines of one direction from one site: grid x over (tile, parity)
+// pairs, y = d; lengths 1..T (d = 0) or 1..R
+template <typename T>
+__global__ void __launch_bounds__(BLOCK) k_lines(Planes<T> U, Lines<T> Ls, Geo g)
+{
+    int p;
+    const uint32_t i = tile_base_pair(g, p) + threadIdx.x;
+    if (i >= g.vh) return;
+    const int d = (int)blockIdx.y;
+    const int nmax = (d == 0) ? Ls.Tl : Ls.R;
+    const Site z = cb_site(g, i, p);
+    Site w = z;
+    Q<T> acc = link(U, g, d, w);
+    for (int n = 1; n <= nmax; n++) {
+        if (n > 1) acc = qmul(acc, link(U, g, d, w));
+        const size_t idx = (d == 0) ? (size_t)(n - 1) : (size_t)(Ls.Tl + (d - 1) * Ls.R + n - 1);
+        typename Store<T>::V v;
+        v.x = acc.c0; v.y = acc.c1; v.z = acc.c2; v.w = acc.c3;
+        Ls.base[idx * 2 * (size_t)g.vh + (size_t)p * g.vh + i] = v;
+        w = shift(g, w, d, 1);
     }
 }
 
 // grid: x over (tile, parity) pairs, y = mu
 template <typename T>
-__global__ void __launch_bounds__(BLOCK) k_wloop_grad(Planes<T> U, Geo g, int R, int Tt, Planes<T> G)
+__global__ void __launch_bounds__(BLOCK) k_wloop_grad(Lines<T> Ls, Geo g, int R, int Tt, Planes<T> G)
 {
 #pragma clang fp contract(fast)
     int p;
     uint32_t i = tile_base_pair(g, p) + threadIdx.x;
     if (i >= g.vh) return;
-    int mu = blockIdx.y;
-    Site y = cb_site(g, i, p);
+    const int mu = blockIdx.y;
+    const Site y = cb_site(g, i, p);
     Q<T> grad = qzero<T>();
     if (mu == 0) {
+        const Site yt = shift(g, y, 0, 1);
         for (int d = 1; d < 4; d++) {
             for (int k = 0; k < Tt; k++) {
-                // left edge, position k: forward link
-                Q<T> acc = qunit<T>();
-                Site s = shift(g, y, 0, 1);
-                seg(U, g, acc, s, 0, Tt - 1 - k, false);
-                seg(U, g, acc, s, d, R, false);
-                seg(U, g, acc, s, 0, Tt, true);
-                seg(U, g, acc, s, d, R, true);
-                seg(U, g, acc, s, 0, k, false);
-                grad = qadd(grad, qdag(acc));
+                // left edge, position k (loop from x = y - k t): the link runs forward
+                const Site x = shiftn(g, y, 0, -k);
+                Q<T> t = line(Ls, g, 0, Tt - 1 - k, yt);
+                t = qmul(t, line(Ls, g, d, R, shiftn(g, x, 0, Tt)));
+                t = qmul_dag(t, line(Ls, g, 0, Tt, shiftn(g, x, d, R)));
+                t = qmul_dag(t, line(Ls, g, d, R, x));
+                t = qmul(t, line(Ls, g, 0, k, x));
+                grad = qadd(grad, qdag(t));
                 // right edge, position k: the loop runs the link backwards
-                acc = qunit<T>();
-                s = y;
-                seg(U, g, acc, s, 0, Tt - 1 - k, true);
-                seg(U, g, acc, s, d, R, true);
-                seg(U, g, acc, s, 0, Tt, false);
-                seg(U, g, acc, s, d, R, false);
-                seg(U, g, acc, s, 0, k, true);
-                grad = qadd(grad, acc);
+                const Site z1 = shiftn(g, y, 0, -(Tt - 1 - k));
+                const Site z2 = shiftn(g, z1, d, -R);
+                t = qdag(line(Ls, g, 0, Tt - 1 - k, z1));
+                t = qmul_dag(t, line(Ls, g, d, R, z2));
+                t = qmul(t, line(Ls, g, 0, Tt, z2));
+                t = qmul(t, line(Ls, g, d, R, shiftn(g, z2, 0, Tt)));
+                t = qmul_dag(t, line(Ls, g, 0, k, yt));
+                grad = qadd(grad, t);
             }
         }
     } else {
         const int d = mu;
+        const Site yd = shift(g, y, d, 1);
         for (int k = 0; k < R; k++) {
-            // top edge, position k: forward link
-            Q<T> acc = qunit<T>();
-            Site s = shift(g, y, d, 1);
-            seg(U, g, acc, s, d, R - 1 - k, false);
-            seg(U, g, acc, s, 0, Tt, true);
-            seg(U, g, acc, s, d, R, true);
-            seg(U, g, acc, s, 0, Tt, false);
-            seg(U, g, acc, s, d, k, false);
-            grad = qadd(grad, qdag(acc));
+            // top edge, position k: forward
+            const Site b = shiftn(g, shiftn(g, y, d, -k), 0, -Tt);
+            Q<T> t = line(Ls, g, d, R - 1 - k, yd);
+            t = qmul_dag(t, line(Ls, g, 0, Tt, shiftn(g, b, d, R)));
+            t = qmul_dag(t, line(Ls, g, d, R, b));
+            t = qmul(t, line(Ls, g, 0, Tt, b));
+            t = qmul(t, line(Ls, g, d, k, shiftn(g, y, d, -k)));
+            grad = qadd(grad, qdag(t));
             // bottom edge, position k: backwards
-            acc = qunit<T>();
-            s = y;
-            seg(U, g, acc, s, d, R - 1 - k, true);
-            seg(U, g, acc, s, 0, Tt, false);
-            seg(U, g, acc, s, d, R, false);
-            seg(U, g, acc, s, 0, Tt, true);
-            seg(U, g, acc, s, d, k, true);
-            grad = qadd(grad, acc);
+            const Site c = shiftn(g, y, d, -(R - 1 - k));
+            t = qdag(line(Ls, g, d, R - 1 - k, c));
+            t = qmul(t, line(Ls, g, 0, Tt, c));
+            t = qmul(t, line(Ls, g, d, R, shiftn(g, c, 0, Tt)));
+            t = qmul_dag(t, line(Ls, g, 0, Tt, shiftn(g, y, d, k + 1)));
+            t = qmul_dag(t, line(Ls, g, d, k, yd));
+            grad = qadd(grad, t);
         }
     }
     qstore<T>(G.p[mu][p], i, qscale(grad, T(2)));
