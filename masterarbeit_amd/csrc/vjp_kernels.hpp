@@ -68,7 +68,7 @@ __global__ void __launch_bounds__(BLOCK) k_vjp_tangent(Planes<T> U, Planes<T> G,
 // and G_b; for every b those around y-b, which close the backward staples of
 // the G_a, a != b.  A batch's loads are issued together before its products
 // (its latency is exposed once) and only one batch is live at a time
-// (spatial FP64: 166 VGPRs, three waves per SIMD).  A term the mode does not
+// (spatial FP64: 160 VGPRs, three waves per SIMD).  A term the mode does not
 // have is neither loaded nor computed; the spatial mode's arithmetic is the
 // same operation for operation as with every term present.
 // grid: x over (tile, parity) pairs
