@@ -180,3 +180,23 @@ def test_vjp_400_steps_48_finite_differences(lgt):
             worst = max(worst, rel)
             assert rel < 1e-6 or abs(an - fd) < 1e-9, (n, mu, an, fd)
         print(f"48^4 APE-400 adjoint vs finite differences: worst relative deviation {worst:.2e}")
+
+
+def test_vjp_rejects_bad_calls(lgt):
+    """Argument and order errors come back as errors, not as results."""
+    L = (4, 4, 4, 4)
+    with lgt.Engine(L, 0, 1) as e:
+        e.init_field(1)
+        with pytest.raises(lgt.LgtError):
+            e.ape_vjp_backward(np.zeros((e.vol, 4, 4)))          # no forward pass yet
+        for bad in (dict(nsteps=0, sp=0.25), dict(nsteps=3, sp=0.25, mode=3), dict(nsteps=3, sp=0.25, segment=-1)):
+            with pytest.raises(lgt.LgtError):
+                e.ape_vjp_forward(**bad)
+        e.ape_vjp_forward(3, 0.25)
+        with pytest.raises(lgt.LgtError):
+            e.ape_vjp_backward()                                  # no cotangent: neither passed nor left by wloop_grad
+        with pytest.raises(lgt.LgtError):
+            e.wloop_grad(1, L[0])                                 # T must stay below the time extent
+        e.wloop_grad(1, 2, host=False)
+        G = e.ape_vjp_backward()
+        assert np.isfinite(G).all()
