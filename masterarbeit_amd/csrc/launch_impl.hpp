@@ -105,12 +105,31 @@ hipError_t Launch<T>::plaquette(hipStream_t s, const Geo& g, const RawPlanes& P,
     return hipGetLastError();
 }
 
+// FP32 spatial smearing runs the site-pair kernel (the one case it is faster
+// in) when the output's parity planes are adjacent; returns false where
+// k_ape serves
+template <typename T>
+static bool ape_zpair(hipStream_t s, const Geo& g, const Planes<T>& Z, const Planes<T>& O, int mode, double sp,
+                      const NormPlanes<T>& mn)
+{
+    if constexpr (sizeof(T) == 4) {
+        bool adjacent = true;
+        for (int mu = 1; mu < 4; mu++) adjacent = adjacent && O.p[mu][1] == O.p[mu][0] + g.vh;
+        if (mode == 0 && adjacent) {
+            hipLaunchKernelGGL((k_ape_zpair<T, 0>), dim3(grid_x(g)), dim3(BLOCK), 0, s, Z, O, g, (T)sp, mn);
+            return true;
+        }
+    }
+    return false;
+}
+
 template <typename T>
 hipError_t Launch<T>::ape(hipStream_t s, const Geo& g, const RawPlanes& in, const RawPlanes& out, int mode, double sp)
 {
     dim3 grid(2 * grid_x(g)), block(BLOCK);
     Planes<T> Z = planes<T>(in), O = planes<T>(out);
     NormPlanes<T> none{};
+    if (ape_zpair<T>(s, g, Z, O, mode, sp, none)) return hipGetLastError();
     if (mode == 0) hipLaunchKernelGGL((k_ape<T, 0>), grid, block, 0, s, Z, O, g, (T)sp, none);
     else if (mode == 1) hipLaunchKernelGGL((k_ape<T, 1>), grid, block, 0, s, Z, O, g, (T)sp, none);
     else hipLaunchKernelGGL((k_ape<T, 2>), grid, block, 0, s, Z, O, g, (T)sp, none);
@@ -160,6 +179,7 @@ hipError_t Launch<T>::ape_save(hipStream_t s, const Geo& g, const RawPlanes& in,
 {
     dim3 grid(2 * grid_x(g)), block(BLOCK);
     Planes<T> Z = planes<T>(in), O = planes<T>(out);
+    if (ape_zpair<T>(s, g, Z, O, mode, sp, norms<T>(mn))) return hipGetLastError();
     if (mode == 0) hipLaunchKernelGGL((k_ape<T, 0>), grid, block, 0, s, Z, O, g, (T)sp, norms<T>(mn));
     else if (mode == 1) hipLaunchKernelGGL((k_ape<T, 1>), grid, block, 0, s, Z, O, g, (T)sp, norms<T>(mn));
     else hipLaunchKernelGGL((k_ape<T, 2>), grid, block, 0, s, Z, O, g, (T)sp, norms<T>(mn));
