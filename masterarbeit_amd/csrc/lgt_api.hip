@@ -1204,7 +1204,11 @@ int lgt_ape_vjp_backward(lgt_engine* e, const double* cotangent, double* grad)
         const int n0 = s * v.seg, n1 = std::min(n0 + v.seg, v.nsteps);
         if (!(s == v.nck - 1 && v.top_saved)) {
             v.top_saved = false;
-            for (int n = n0; n < n1; n++) {
+            // below the top segment the last step's output and |M| are not
+            // read (the next segment starts from its checkpoint and the |M|
+            // the forward kept), so that step is not recomputed
+            const int n_end = (s == v.nck - 1) ? n1 : n1 - 1;
+            for (int n = n0; n < n_end; n++) {
                 const RawPlanes& in = (n == n0) ? v.ckP[s] : v.segUP[n - n0 - 1];
                 TRY(run(e, LGT_K_APE_SAVE, "ape recompute", [&] {
                     return DISPATCH(e, ape_save(e->stream, ugeo(e, LGT_K_APE_SAVE, 0), in, v.segUP[n - n0], v.segMN[n - n0],
