@@ -140,26 +140,27 @@ def time_stream(e, prec, L, reps=10):
     return {"bytes_per_launch": nbytes, "kernel_ms_avg": ms / n, "achieved_GBs": nbytes / (ms / n * 1e-3) / 1e9}
 
 
-def time_vjp(lgt, L, prec, seed, beta, nsteps=400, sp=0.25, R=5, T=5, therm=50, mode=0):
+def time_vjp(lgt, L, prec, seed, beta, nsteps=400, sp=0.25, R=5, T=5, therm=50, mode=0, segment=0):
     """Adjoint of N APE steps (spatial: alpha = 0.5, sp = 0.25; mode 1: all
     directions) with the loop objective L = sum_x sum_d 2 Re W_{RxT}: forward
-    with sqrt(N) checkpoints, then the backward pass (recompute + adjoint
-    steps) for the objective's gradient.  The ratio is backward / forward wall
-    time."""
+    with sqrt(N) checkpoints (segment 0) or checkpoints every `segment` steps
+    (segment = N: every step's output kept, nothing recomputed), then the
+    backward pass (recompute + adjoint steps) for the objective's gradient.
+    The ratio is backward / forward wall time."""
     e = lgt.Engine(L, prec, seed)
     e.init_field(1)
     e.update(therm, 1, beta)
     U0 = e.download()
     # one untimed pass allocates the checkpoint and segment buffers (kept by
     # the engine for later passes of the same layout)
-    e.ape_vjp_forward(nsteps, sp, mode=mode)
+    e.ape_vjp_forward(nsteps, sp, segment, mode=mode)
     e.wloop_grad(R, T, host=False)
     e.ape_vjp_backward(grad=False)
     e.upload(U0)
     del U0
     e.sync()
     t0 = time.perf_counter()
-    e.ape_vjp_forward(nsteps, sp, mode=mode)
+    e.ape_vjp_forward(nsteps, sp, segment, mode=mode)
     e.sync()
     t_fwd = time.perf_counter() - t0
     t0 = time.perf_counter()
@@ -170,7 +171,7 @@ def time_vjp(lgt, L, prec, seed, beta, nsteps=400, sp=0.25, R=5, T=5, therm=50, 
     e.ape_vjp_backward(grad=False)
     e.sync()
     t_bwd = time.perf_counter() - t0
-    out = {"mode": ("spatial", "4d", "temporal")[mode], "nsteps": nsteps, "segment": int(np.ceil(np.sqrt(nsteps))), "forward_s": t_fwd, "backward_s": t_bwd,
+    out = {"mode": ("spatial", "4d", "temporal")[mode], "nsteps": nsteps, "segment": segment or int(np.ceil(np.sqrt(nsteps))), "forward_s": t_fwd, "backward_s": t_bwd,
            "objective_grad_s": t_obj, "vjp_over_forward": t_bwd / t_fwd, "device_bytes": e.device_bytes()}
     e.close()
     return out
@@ -401,6 +402,11 @@ def main():
     vjp64 = time_vjp(lgt, L, lgt.FP64, args.seed, args.beta, nsteps=args.vjp_steps)
     log(f"[bench] fp64 VJP/forward = {vjp64['vjp_over_forward']:.2f} (forward {vjp64['forward_s']:.3f} s)")
     vjp32 = time_vjp(lgt, L, lgt.FP32, args.seed, args.beta, nsteps=args.vjp_steps)
+    # the same adjoint with every step's output kept in HBM (48^4 FP32, 400
+    # steps: 128 GB), so that the backward recomputes nothing
+    vjp32_all = time_vjp(lgt, L, lgt.FP32, args.seed, args.beta, nsteps=args.vjp_steps, segment=args.vjp_steps)
+    log(f"[bench] fp32 VJP/forward = {vjp32['vjp_over_forward']:.2f}, all states kept: "
+        f"{vjp32_all['vjp_over_forward']:.2f} ({vjp32_all['device_bytes'] / 1e9:.0f} GB)")
     vjp4d64 = time_vjp(lgt, L, lgt.FP64, args.seed, args.beta, nsteps=args.vjp_steps, sp=1.0 / 6.0, mode=1)
     log(f"[bench] fp64 4D APE VJP/forward = {vjp4d64['vjp_over_forward']:.2f}")
 
@@ -487,6 +493,7 @@ def main():
         "plaquette_kernel": {"fp32": plaq32, "fp64": plaq64},
         "ape_vjp": {"fp64": vjp64, "fp32": vjp32},
         "ape_vjp_4d": {"fp64": vjp4d64},
+        "ape_vjp_all_states": {"fp32": vjp32_all},
         "stream_pass": stream32,
         "observables_s": {"fp32": obs32, "fp64": obs64},
         "device_bytes": {"fp32": r32["device_bytes"], "fp64": r64["device_bytes"]},

@@ -201,10 +201,13 @@ int lgt_qtop(lgt_engine *e, int kind, double *q, double *abs_sum, double *densit
  *
  * lgt_ape_vjp_forward smears the current field like lgt_ape(mode, nsteps,
  * sp) and keeps a checkpoint of the links the mode changes every `segment`
- * steps (0: ceil(sqrt(nsteps))).  lgt_ape_vjp_backward then maps a cotangent
- * dL/dU of the smeared field to dL/dU of the thin field, recomputing each
- * segment's steps from its checkpoint (storing every step's |U + sp S|) and
- * running the adjoint steps backwards through it.  Cotangents and gradients
+ * steps (0: ceil(sqrt(nsteps))), and every step of the last segment.
+ * lgt_ape_vjp_backward then maps a cotangent dL/dU of the smeared field to
+ * dL/dU of the thin field, recomputing each lower segment's steps from its
+ * checkpoint (storing every step's |U + sp S|) and running the adjoint steps
+ * backwards through it.  segment = nsteps keeps every step's output, so
+ * nothing is recomputed (device memory: nsteps fields of the changed links
+ * and their norms).  Cotangents and gradients
  * have 4 real components per link, Euclidean (<a,b> = a0 b0 + a.b), in the
  * field layout of lgt_upload; links the mode does not smear keep their
  * cotangent and add what they contribute to the smeared links' staples (the
