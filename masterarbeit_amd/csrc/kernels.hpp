@@ -870,73 +870,6 @@ __global__ void __launch_bounds__(WL<T>::THREADS) k_wloops(Planes<T> P, Planes<T
 // ---------------------------------------------------------------------------
 // improved clover charge (LGT_Code/modules/obs/topologicalcharge.c:18-199)
 // ---------------------------------------------------------------------------
-// computeLine: l steps along +dir (t = t U) or -dir (t = t U^+), starting from
-// the identity when `first` (1*U = U exactly).
-template <typename T>
-__device__ __forceinline__ void walk(const Planes<T>& P, const Geo& g, Q<T>& t, Site& n, int dir, int l, bool back,
-                                     bool& first)
-{
-    for (int i = 0; i < l; i++) {
-        if (!back) {
-            Q<T> u = link(P, g, dir, n);
-            if (first) t = u;
-            else t = qmul(t, u);
-            n = shift(g, n, dir, 1);
-        } else {
-            n = shift(g, n, dir, -1);
-            Q<T> u = link(P, g, dir, n);
-            if (first) t = qdag(u);
-            else t = qmul_dag(t, u);
-        }
-        first = false;
-    }
-}
-
-// computeCloverAverages(u, x, d1, d2, m, m): four m x m leaves / 4
-template <typename T>
-__device__ __forceinline__ Q<T> clover_average(const Planes<T>& P, const Geo& g, const Site& x, int d1, int d2, int m)
-{
-    Q<T> sum, t;
-    Site n = x;
-    bool f;
-    // each leaf's product is complete before the next leaf's links are loaded:
-    // one leaf (at most 12 links) live at a time keeps the charge kernel at
-    // several waves per SIMD instead of one
-    // +d1,+d2
-    f = true;
-    pin(n);
-    walk(P, g, t, n, d1, m, false, f); walk(P, g, t, n, d2, m, false, f);
-    walk(P, g, t, n, d1, m, true, f);  walk(P, g, t, n, d2, m, true, f);
-    sum = t;
-    pin(sum);
-    asm volatile("" ::: "memory");
-    // +d1,-d2
-    f = true;
-    pin(n);
-    walk(P, g, t, n, d2, m, true, f);  walk(P, g, t, n, d1, m, false, f);
-    walk(P, g, t, n, d2, m, false, f); walk(P, g, t, n, d1, m, true, f);
-    sum = qadd(sum, t);
-    pin(sum);
-    asm volatile("" ::: "memory");
-    // -d1,+d2
-    f = true;
-    pin(n);
-    walk(P, g, t, n, d2, m, false, f); walk(P, g, t, n, d1, m, true, f);
-    walk(P, g, t, n, d2, m, true, f);  walk(P, g, t, n, d1, m, false, f);
-    sum = qadd(sum, t);
-    pin(sum);
-    asm volatile("" ::: "memory");
-    // -d1,-d2
-    f = true;
-    pin(n);
-    walk(P, g, t, n, d1, m, true, f);  walk(P, g, t, n, d2, m, true, f);
-    walk(P, g, t, n, d1, m, false, f); walk(P, g, t, n, d2, m, false, f);
-    sum = qadd(sum, t);
-    pin(sum);
-    asm volatile("" ::: "memory");
-    return qdiv(sum, T(4));
-}
-
 // (X - X^+)/2 (topologicalcharge.c:112-121)
 template <typename T>
 __device__ __forceinline__ Q<T> anti_hermitian(Q<T> X)
@@ -946,25 +879,139 @@ __device__ __forceinline__ Q<T> anti_hermitian(Q<T> X)
     return qdiv(X, T(2));
 }
 
-template <typename T>
-__device__ __forceinline__ Q<T> improved_F(const Planes<T>& P, const Geo& g, const Site& x, int d1, int d2)
+// A site walked along the two directions of one plane: its coordinates there,
+// its cb index and parity.  Each step updates them in place (compares and
+// selects, no multiplications); recomputing the cb index from the
+// coordinates at every one of the charge's 576 loads per site cost as many
+// instructions as the products (FP32 charge 7% slower).
+template <int D1, int D2> struct PlaneCursor {
+    int c[2];      // coordinates along D1, D2
+    uint32_t cb;   // cb index
+    int par;       // parity
+};
+
+template <int D1, int D2>
+__device__ __forceinline__ void pin(PlaneCursor<D1, D2>& n)
 {
+    asm volatile("" : "+v"(n.c[0]), "+v"(n.c[1]), "+v"(n.cb), "+v"(n.par));
+}
+
+// one step along +-D (D = D1 or D2) with periodic wrap
+template <int D1, int D2, int D>
+__device__ __forceinline__ void cstep(const Geo& g, PlaneCursor<D1, D2>& n, bool back)
+{
+    constexpr int k = (D == D1) ? 0 : 1;
+    const int L = g.L[D];
+    int c = n.c[k];
+    if (D == 3) {
+        // z: the cb index holds z/2
+        const uint32_t hz1 = (uint32_t)g.hz - 1u;
+        if (!back) {
+            n.cb += (c & 1) ? ((c + 1 == L) ? (uint32_t)0 - hz1 : 1u) : 0u;
+            c = (c + 1 == L) ? 0 : c + 1;
+        } else {
+            n.cb += (c & 1) ? 0u : ((c == 0) ? hz1 : (uint32_t)0 - 1u);
+            c = (c == 0) ? L - 1 : c - 1;
+        }
+    } else {
+        const uint32_t st = g.stride[D], wrap = (uint32_t)(L - 1) * st;
+        if (!back) {
+            n.cb += (c + 1 == L) ? (uint32_t)0 - wrap : st;
+            c = (c + 1 == L) ? 0 : c + 1;
+        } else {
+            n.cb += (c == 0) ? wrap : (uint32_t)0 - st;
+            c = (c == 0) ? L - 1 : c - 1;
+        }
+    }
+    n.c[k] = c;
+    n.par ^= 1;
+}
+
+template <typename T, int D1, int D2>
+__device__ __forceinline__ Q<T> clink(const Planes<T>& P, const Geo& g, int dir, const PlaneCursor<D1, D2>& n)
+{
+    return link_at(P, dir, ((uint32_t)n.par * g.vh + n.cb) * (uint32_t)sizeof(typename Store<T>::V));
+}
+
+// computeLine: l steps along +D (t = t U) or -D (t = t U^+), starting from
+// the identity when `first` (1*U = U exactly)
+template <typename T, int D1, int D2, int D>
+__device__ __forceinline__ void cwalk(const Planes<T>& P, const Geo& g, Q<T>& t, PlaneCursor<D1, D2>& n, int l,
+                                      bool back, bool& first)
+{
+    for (int i = 0; i < l; i++) {
+        if (!back) {
+            Q<T> u = clink(P, g, D, n);
+            if (first) t = u;
+            else t = qmul(t, u);
+            cstep<D1, D2, D>(g, n, false);
+        } else {
+            cstep<D1, D2, D>(g, n, true);
+            Q<T> u = clink(P, g, D, n);
+            if (first) t = qdag(u);
+            else t = qmul_dag(t, u);
+        }
+        first = false;
+    }
+}
+
+// computeCloverAverages(u, x, D1, D2, m, m): four m x m leaves / 4.  Each
+// leaf's product is complete before the next leaf's links are loaded: one
+// leaf (at most 12 links) live at a time keeps the charge kernel at several
+// waves per SIMD instead of one.
+template <typename T, int D1, int D2>
+__device__ __forceinline__ Q<T> clover_average(const Planes<T>& P, const Geo& g, PlaneCursor<D1, D2> n, int m)
+{
+    Q<T> sum, t;
+    bool f;
+    // +D1,+D2
+    f = true;
+    pin(n);
+    cwalk<T, D1, D2, D1>(P, g, t, n, m, false, f); cwalk<T, D1, D2, D2>(P, g, t, n, m, false, f);
+    cwalk<T, D1, D2, D1>(P, g, t, n, m, true, f);  cwalk<T, D1, D2, D2>(P, g, t, n, m, true, f);
+    sum = t;
+    pin(sum);
+    asm volatile("" ::: "memory");
+    // +D1,-D2
+    f = true;
+    pin(n);
+    cwalk<T, D1, D2, D2>(P, g, t, n, m, true, f);  cwalk<T, D1, D2, D1>(P, g, t, n, m, false, f);
+    cwalk<T, D1, D2, D2>(P, g, t, n, m, false, f); cwalk<T, D1, D2, D1>(P, g, t, n, m, true, f);
+    sum = qadd(sum, t);
+    pin(sum);
+    asm volatile("" ::: "memory");
+    // -D1,+D2
+    f = true;
+    pin(n);
+    cwalk<T, D1, D2, D2>(P, g, t, n, m, false, f); cwalk<T, D1, D2, D1>(P, g, t, n, m, true, f);
+    cwalk<T, D1, D2, D2>(P, g, t, n, m, true, f);  cwalk<T, D1, D2, D1>(P, g, t, n, m, false, f);
+    sum = qadd(sum, t);
+    pin(sum);
+    asm volatile("" ::: "memory");
+    // -D1,-D2
+    f = true;
+    pin(n);
+    cwalk<T, D1, D2, D1>(P, g, t, n, m, true, f);  cwalk<T, D1, D2, D2>(P, g, t, n, m, true, f);
+    cwalk<T, D1, D2, D1>(P, g, t, n, m, false, f); cwalk<T, D1, D2, D2>(P, g, t, n, m, false, f);
+    sum = qadd(sum, t);
+    pin(sum);
+    asm volatile("" ::: "memory");
+    return qdiv(sum, T(4));
+}
+
+// F_{D1 D2} = AH(1.5 C1 - 0.15 C2 + C3/90) at the site s (cb index i,
+// parity p), topologicalcharge.c:123-162
+template <typename T, int D1, int D2>
+__device__ __forceinline__ Q<T> improved_F(const Planes<T>& P, const Geo& g, const Site& s, uint32_t i, int p)
+{
+    const PlaneCursor<D1, D2> x{{s.c[D1], s.c[D2]}, i, p};
     const T k1 = T(1.5), k2 = T(-0.15), k3 = T(1.) / T(90.);
-    Q<T> U1 = qscale(clover_average(P, g, x, d1, d2, 1), k1);
-    Q<T> U2 = qscale(clover_average(P, g, x, d1, d2, 2), k2);
-    Q<T> U3 = qscale(clover_average(P, g, x, d1, d2, 3), k3);
+    Q<T> U1 = qscale(clover_average<T, D1, D2>(P, g, x, 1), k1);
+    Q<T> U2 = qscale(clover_average<T, D1, D2>(P, g, x, 2), k2);
+    Q<T> U3 = qscale(clover_average<T, D1, D2>(P, g, x, 3), k3);
     Q<T> F = qadd(U1, U2);
     F = qadd(F, U3);
     return anti_hermitian(F);
-}
-
-template <typename T>
-__device__ __forceinline__ T improved_products(const Planes<T>& P, const Geo& g, const Site& x, int d1, int d2, int d3,
-                                               int d4)
-{
-    Q<T> F = improved_F(P, g, x, d1, d2);
-    Q<T> G = improved_F(P, g, x, d3, d4);
-    return T(2) * qmul(F, G).c0;
 }
 
 // 1x1 leaves (topologicalcharge.c:201-290)
@@ -1059,8 +1106,9 @@ __global__ void __launch_bounds__(BLOCK) k_qtop(Planes<T> P, Geo g, DD* __restri
         Site s = cb_site(g, i, p);
         T v;
         if (KIND == 0)
-            v = T(8.) * improved_products(P, g, s, 0, 1, 2, 3) + T(8.) * improved_products(P, g, s, 0, 2, 3, 1) +
-                T(8.) * improved_products(P, g, s, 0, 3, 1, 2);
+            v = T(8.) * (T(2) * qmul(improved_F<T, 0, 1>(P, g, s, i, p), improved_F<T, 2, 3>(P, g, s, i, p)).c0) +
+                T(8.) * (T(2) * qmul(improved_F<T, 0, 2>(P, g, s, i, p), improved_F<T, 3, 1>(P, g, s, i, p)).c0) +
+                T(8.) * (T(2) * qmul(improved_F<T, 0, 3>(P, g, s, i, p), improved_F<T, 1, 2>(P, g, s, i, p)).c0);
         else
             v = -clover_products_1x1(P, g, s, 0, 1, 3, 2) - clover_products_1x1(P, g, s, 1, 2, 3, 0) -
                 clover_products_1x1(P, g, s, 2, 0, 3, 1);

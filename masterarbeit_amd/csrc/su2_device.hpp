@@ -204,13 +204,19 @@ template <typename T> struct Planes {
     typename Store<T>::V* p[4][2];
 };
 
+// the link at byte offset off from the start of direction mu's planes
 template <typename T>
-__device__ __forceinline__ Q<T> link(const Planes<T>& P, const Geo& g, int mu, const Site& s)
+__device__ __forceinline__ Q<T> link_at(const Planes<T>& P, int mu, uint32_t off)
 {
-    const uint32_t off = ((uint32_t)parity(s) * g.vh + cb_index(g, s)) * (uint32_t)sizeof(typename Store<T>::V);
     const typename Store<T>::V* q = (const typename Store<T>::V*)((const char*)P.p[mu][0] + off);
     typename Store<T>::V v = *q;
     return Q<T>{v.x, v.y, v.z, v.w};
+}
+
+template <typename T>
+__device__ __forceinline__ Q<T> link(const Planes<T>& P, const Geo& g, int mu, const Site& s)
+{
+    return link_at(P, mu, ((uint32_t)parity(s) * g.vh + cb_index(g, s)) * (uint32_t)sizeof(typename Store<T>::V));
 }
 
 // ---------------------------------------------------------------------------
