@@ -337,7 +337,8 @@ def test_update_matches_oracle_chain(lgt, L, n_or):
 def test_smallest_extents_match_oracle(lgt, L):
     """Extent 2 (x+1 and x-1 are the same site) and lattices smaller than one
     256-site tile: sweeps, every APE mode, plaquette, loops up to T = L0 - 1
-    with wrapping R, and both charges against the oracle."""
+    with wrapping R, and both charges against the oracle; FP32 smearing (the
+    site-pair kernel in spatial smearing) and charges within FP32 rounding."""
     seed = 11
     o, U = thermal_field(L, seed=4, nsweeps=1)
     V = U.copy()
@@ -363,6 +364,16 @@ def test_smallest_extents_match_oracle(lgt, L):
             assert abs(q - q_ref) <= 1e-13 * s_ref
     with engine(lgt, L, 1, seed=seed, U=V) as e32:
         assert abs(e32.plaquette() - o.plaquette(V, acc=True)) < 1e-6
+        for mode, sp in ((0, 0.25), (1, 1.0 / 6.0), (2, 0.25)):
+            e32.upload(V)
+            W = V.copy()
+            e32.ape(mode, 3, sp)
+            o.ape(W, mode, 3, sp)
+            assert np.max(np.abs(e32.download() - W)) <= 2e-6
+        for kind in (0, 1):
+            q, s_abs = e32.qtop(kind)
+            q_ref, s_ref = o.qtop(W, "improved" if kind == 0 else "clover")[:2]
+            assert abs(q - q_ref) <= 1e-5 * s_ref
         e32.update(2, 1, 3.09)
         np.testing.assert_allclose(np.linalg.norm(e32.download(), axis=-1), 1.0, atol=1e-6)
 
