@@ -815,37 +815,47 @@ __global__ void __launch_bounds__(WL<T>::THREADS) k_wloops(Planes<T> P, Planes<T
     const int d = 1 + (int)blockIdx.y;
     const int R = grp.R;
     const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
-    Site s0[NS], sR[NS];
+    // the lines' top ends x + t0 t and x + R d + t0 t as time coordinate and cb
+    // index, advanced one time step per T (t0: their time offset, mod L0)
+    int tT[NS], tR[NS];
+    uint32_t cbT[NS], cbR[NS];
     Q<T> SR[NS], Lx[NS], LxR[NS];
     bool valid[NS];
+    const int pR = p ^ (R & 1);   // parity of x + R d
+    const uint32_t st0 = g.stride[0], wrap0 = (uint32_t)(g.L[0] - 1) * st0;
+    constexpr uint32_t LB = (uint32_t)sizeof(typename Store<T>::V);
 #pragma unroll
     for (int j = 0; j < NS; j++) {
         const uint32_t i = base + (uint32_t)(threadIdx.x + NT * j);
         valid[j] = i < g.vh;
-        s0[j] = cb_site(g, valid[j] ? i : 0u, p);
-        sR[j] = shift(g, s0[j], d, R % g.L[d]);
-        SR[j] = link(S, g, d, s0[j]);
-        Lx[j] = link(P, g, 0, s0[j]);
-        LxR[j] = link(P, g, 0, sR[j]);
+        const Site s0 = cb_site(g, valid[j] ? i : 0u, p);
+        const Site sR = shift(g, s0, d, R % g.L[d]);
+        tT[j] = s0.c[0];
+        cbT[j] = valid[j] ? i : 0u;
+        tR[j] = sR.c[0];
+        cbR[j] = cb_index(g, sR);
+        SR[j] = link(S, g, d, s0);
+        Lx[j] = link(P, g, 0, s0);
+        LxR[j] = link(P, g, 0, sR);
     }
-    int t0 = 0;   // time offset of the lines' top ends, modulo L0
+    int t0 = 0;
     for (int Tt = 1; Tt <= grp.Tmax; Tt++) {
         const int slot = slots[grp.slot_off + Tt - 1];
+        const uint32_t odd = (uint32_t)(t0 & 1);
         double acc = 0.0;
 #pragma unroll
         for (int j = 0; j < NS; j++) {
-            Site top = s0[j], topR = sR[j];
-            top.c[0] += t0;
-            top.c[0] -= (top.c[0] >= g.L[0]) ? g.L[0] : 0;
-            topR.c[0] += t0;
-            topR.c[0] -= (topR.c[0] >= g.L[0]) ? g.L[0] : 0;
             if (Tt > 1) {
-                Lx[j] = qmul(Lx[j], link(P, g, 0, top));
-                LxR[j] = qmul(LxR[j], link(P, g, 0, topR));
+                Lx[j] = qmul(Lx[j], link_at(P, 0, (((uint32_t)p ^ odd) * g.vh + cbT[j]) * LB));
+                LxR[j] = qmul(LxR[j], link_at(P, 0, (((uint32_t)pR ^ odd) * g.vh + cbR[j]) * LB));
             }
+            // one step in t
+            cbT[j] += (tT[j] + 1 == g.L[0]) ? (uint32_t)0 - wrap0 : st0;
+            tT[j] = (tT[j] + 1 == g.L[0]) ? 0 : tT[j] + 1;
+            cbR[j] += (tR[j] + 1 == g.L[0]) ? (uint32_t)0 - wrap0 : st0;
+            tR[j] = (tR[j] + 1 == g.L[0]) ? 0 : tR[j] + 1;
             if (slot >= 0) {
-                top = shift(g, top, 0, 1);
-                const Q<T> St = link(S, g, d, top);
+                const Q<T> St = link_at(S, d, (((uint32_t)p ^ odd ^ 1u) * g.vh + cbT[j]) * LB);
                 const T w = T(2) * qdot(qmul(Lx[j], St), qmul(SR[j], LxR[j]));
                 acc += valid[j] ? (double)w : 0.0;
             }
