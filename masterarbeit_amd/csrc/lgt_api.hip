@@ -106,6 +106,7 @@ struct lgt_engine {
     size_t staging_bytes = 0;
     int profile = 0;
     int order = -1;   // tile order of every kernel; -1: per kernel (see kgeo)
+    int group = 0;    // LGT_OPT_GROUP (0: per kernel, see kgeo)
     int alternate = 1;  // chain launches alternate between forward and reversed tile order
     uint32_t flip = 0;  // reverses the next chain launch (see ugeo)
     std::vector<ProfEvent> events;
@@ -330,6 +331,17 @@ Geo kgeo(const lgt_engine* e, int cls, int mu)
     (void)mu;
     Geo g = e->geo;
     int order = (e->order >= 0) ? e->order : (cls == LGT_K_WLOOP_GRAD ? 3 : (g.gs ? 5 : 1));
+    // The FP64 loops read 16 time slices ahead of a tile in 32-byte links: a
+    // third of the automatic group keeps that window in L2 (48^4, 160 loops:
+    // 20.4 instead of 26.9 ms; FP32 is fastest with the automatic group).
+    if (cls == LGT_K_WLOOP && e->prec == LGT_FP64 && e->group == 0 && g.gs > 0) {
+        const uint32_t tiles = g.vh / (uint32_t)g.L[0] / 256u;
+        for (uint32_t d = std::max(1u, (uint32_t)g.gs / 3u); d >= 1; d--)
+            if (tiles % d == 0) {
+                set_group(g, (int)d);
+                break;
+            }
+    }
     g.order = tile_order_ok(g, order);
     return g;
 }
@@ -586,6 +598,7 @@ int lgt_set_option(lgt_engine* e, int option, int value)
         return LGT_OK;
     case LGT_OPT_GROUP:
         if (!set_group(e->geo, value)) return fail(LGT_EINVAL, "lgt_set_option: group size %d", value);
+        e->group = value;
         return LGT_OK;
     case LGT_OPT_ALTERNATE:
         if (value != 0 && value != 1) return fail(LGT_EINVAL, "lgt_set_option: alternate %d", value);
