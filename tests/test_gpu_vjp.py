@@ -97,6 +97,30 @@ def test_vjp_modes_share_an_engine(lgt):
             np.testing.assert_array_equal(Gm, G)
 
 
+@pytest.mark.parametrize("prec", [0, 1])
+@pytest.mark.parametrize("mode,sp", MODES)
+def test_vjp_segmentations_agree_bitwise(lgt, prec, mode, sp):
+    """Every checkpoint segmentation runs the same kernels on the same inputs:
+    sqrt(N) checkpoints, uneven segments, every step kept (segment = N) and a
+    repeated backward give the same gradient bit for bit, in both precisions
+    (FP32 spatial smearing through the site-pair kernel)."""
+    L = (8, 8, 8, 8)
+    o, U = thermal(L)
+    N = 9
+    res = []
+    with lgt.Engine(L, prec, 1) as e:
+        for seg in (0, 2, 4, N, 1):
+            e.upload(U)
+            e.ape_vjp_forward(N, sp, seg, mode=mode)
+            e.wloop_grad(2, 3, host=False)
+            res.append(e.ape_vjp_backward())
+        # a second backward of the last forward recomputes its top segment
+        e.wloop_grad(2, 3, host=False)
+        res.append(e.ape_vjp_backward())
+    for G in res[1:]:
+        np.testing.assert_array_equal(G, res[0])
+
+
 @pytest.mark.parametrize("mode,sp", MODES)
 def test_vjp_explicit_cotangent(lgt, mode, sp):
     L = (6, 4, 4, 8)
