@@ -164,28 +164,54 @@ __device__ __forceinline__ uint64_t lex_index(const Geo& g, const Site& s)
     return (((uint64_t)s.c[0] * g.L[1] + s.c[1]) * g.L[2] + s.c[2]) * g.L[3] + s.c[3];
 }
 
-// reference AoS FP64 [VOL][4][4] -> planes
+// Reference AoS FP64 [VOL][4][4] <-> planes, one thread per lexicographic
+// site n: its record (4 links, 128 bytes) is read or written whole, so a wave
+// moves 8 KB of contiguous AoS, and its links are at cb index n/2 of the
+// parity planes (n even and n + 1 hold one site of each parity).  One thread
+// per (link, parity) wrote 32-byte pieces 256 bytes apart instead (the FP32
+// download's conversion ran at 1.2 TB/s).
+// grid: x over lexicographic sites (VOL / BLOCK blocks)
+__device__ __forceinline__ int lex_parity(const Geo& g, uint32_t n)
+{
+    const Site s = cb_site(g, n >> 1, 0);   // t, x, y of n (and z/2 in c[3] >> 1)
+    const int z = (s.c[3] & ~1) | (int)(n & 1u);
+    return (s.c[0] + s.c[1] + s.c[2] + z) & 1;
+}
+
 template <typename T>
 __global__ void __launch_bounds__(BLOCK) k_from_aos(Planes<T> P, Geo g, const double* __restrict__ aos)
 {
-    uint32_t i = blockIdx.x * BLOCK + threadIdx.x;
-    int mu = blockIdx.y >> 1, p = blockIdx.y & 1;
-    if (i >= g.vh) return;
-    uint64_t n = lex_index(g, cb_site(g, i, p));
-    const double* a = aos + 16 * n + 4 * mu;
-    qstore<T>(P.p[mu][p], i, Q<T>{(T)a[0], (T)a[1], (T)a[2], (T)a[3]});
+    typedef double d2v __attribute__((ext_vector_type(2)));
+    const uint32_t n = blockIdx.x * BLOCK + threadIdx.x;
+    if (n >= 2 * g.vh) return;
+    const int p = lex_parity(g, n);
+    const d2v* a = (const d2v*)(aos + 16 * (size_t)n);
+    d2v v[8];
+#pragma unroll
+    for (int k = 0; k < 8; k++) v[k] = a[k];
+#pragma unroll
+    for (int mu = 0; mu < 4; mu++)
+        qstore<T>(P.p[mu][p], n >> 1,
+                  Q<T>{(T)v[2 * mu].x, (T)v[2 * mu].y, (T)v[2 * mu + 1].x, (T)v[2 * mu + 1].y});
 }
 
 template <typename T>
 __global__ void __launch_bounds__(BLOCK) k_to_aos(Planes<T> P, Geo g, double* __restrict__ aos)
 {
-    uint32_t i = blockIdx.x * BLOCK + threadIdx.x;
-    int mu = blockIdx.y >> 1, p = blockIdx.y & 1;
-    if (i >= g.vh) return;
-    uint64_t n = lex_index(g, cb_site(g, i, p));
-    Q<T> q = qload<T>(P.p[mu][p], i);
-    double* a = aos + 16 * n + 4 * mu;
-    a[0] = (double)q.c0; a[1] = (double)q.c1; a[2] = (double)q.c2; a[3] = (double)q.c3;
+    typedef double d2v __attribute__((ext_vector_type(2)));
+    const uint32_t n = blockIdx.x * BLOCK + threadIdx.x;
+    if (n >= 2 * g.vh) return;
+    const int p = lex_parity(g, n);
+    d2v* a = (d2v*)(aos + 16 * (size_t)n);
+#pragma unroll
+    for (int mu = 0; mu < 4; mu++) {
+        const Q<T> q = qload<T>(P.p[mu][p], n >> 1);
+        d2v lo, hi;
+        lo.x = (double)q.c0; lo.y = (double)q.c1;
+        hi.x = (double)q.c2; hi.y = (double)q.c3;
+        a[2 * mu] = lo;
+        a[2 * mu + 1] = hi;
+    }
 }
 
 template <typename T>

@@ -33,14 +33,14 @@ hipError_t Launch<T>::hot(hipStream_t s, const Geo& g, const RawPlanes& P, uint6
 template <typename T>
 hipError_t Launch<T>::from_aos(hipStream_t s, const Geo& g, const RawPlanes& P, const double* aos)
 {
-    hipLaunchKernelGGL(k_from_aos<T>, dim3(grid_x(g), 8), dim3(BLOCK), 0, s, planes<T>(P), g, aos);
+    hipLaunchKernelGGL(k_from_aos<T>, dim3((2 * g.vh + BLOCK - 1) / BLOCK), dim3(BLOCK), 0, s, planes<T>(P), g, aos);
     return hipGetLastError();
 }
 
 template <typename T>
 hipError_t Launch<T>::to_aos(hipStream_t s, const Geo& g, const RawPlanes& P, double* aos)
 {
-    hipLaunchKernelGGL(k_to_aos<T>, dim3(grid_x(g), 8), dim3(BLOCK), 0, s, planes<T>(P), g, aos);
+    hipLaunchKernelGGL(k_to_aos<T>, dim3((2 * g.vh + BLOCK - 1) / BLOCK), dim3(BLOCK), 0, s, planes<T>(P), g, aos);
     return hipGetLastError();
 }
 
