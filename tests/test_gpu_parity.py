@@ -314,6 +314,50 @@ def test_heatbath_subsweep_replay(lgt, L):
                 V = G.copy()                 # continue from the engine's field
 
 
+@pytest.mark.parametrize("prec", [0, 1])
+def test_overrelax_subsweep_replay(lgt, prec):
+    """Each (mu, parity) overrelaxation sub-sweep equals the oracle's
+    sequential one (FP64: to rounding of the closed form; FP32: to FP32
+    rounding, unit links)."""
+    L = (6, 4, 4, 8)
+    o, U = thermal_field(L)
+    with engine(lgt, L, prec, seed=3, U=U) as e:
+        V = e.download()
+        for mu in range(4):
+            for p in range(2):
+                o.or_subsweep(V, mu, p)
+                e.overrelax_subsweep(mu, p)
+                G = e.download()
+                d = np.max(np.abs(G - V))
+                assert d <= (1e-14 if prec == 0 else 2e-6), (mu, p, d)
+                V = G.copy()
+        np.testing.assert_allclose(np.linalg.norm(V, axis=-1), 1.0, atol=1e-14 if prec == 0 else 1e-6)
+
+
+@pytest.mark.parametrize("prec", [0, 1])
+def test_chain_resumes_from_sweep_index(lgt, prec):
+    """The RNG is keyed by the sweep index: a chain stopped after one sweep and
+    resumed on a fresh engine from its field and sweep index reproduces the
+    uninterrupted chain bit for bit."""
+    L = (6, 4, 4, 8)
+    o, U = thermal_field(L)
+    with engine(lgt, L, prec, seed=21, U=U) as e:
+        e.update(3, 1, 3.09)
+        full = e.download()
+        assert e.sweep_index == 3
+    with engine(lgt, L, prec, seed=21, U=U) as e:
+        e.update(1, 1, 3.09)
+        mid = e.download()
+    with engine(lgt, L, prec, seed=21, U=mid) as e:
+        assert e.sweep_index == 0
+        e.sweep_index = 1
+        e.update(2, 1, 3.09)
+        np.testing.assert_array_equal(e.download(), full)
+    with engine(lgt, L, prec, seed=21, U=mid) as e:
+        e.update(2, 1, 3.09)   # resumed at index 0: a different chain
+        assert not np.array_equal(e.download(), full)
+
+
 @pytest.mark.parametrize("L", [(4, 4, 4, 4), (6, 4, 4, 8)])
 @pytest.mark.parametrize("n_or", [0, 1, 3])
 def test_update_matches_oracle_chain(lgt, L, n_or):
