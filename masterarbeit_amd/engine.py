@@ -150,6 +150,13 @@ class Engine:
     def device_bytes(self):
         return self._lib.lgt_device_bytes(self._h)
 
+    def extents(self):
+        """(lattice extents, precision) the engine was created with."""
+        L = (C.c_int * 4)()
+        prec = C.c_int(0)
+        _check(self._lib.lgt_get_extents(self._h, L, C.byref(prec)), "lgt_get_extents")
+        return tuple(L), prec.value
+
     # ---- field
     def init_field(self, start):
         _check(self._lib.lgt_init_field(self._h, start), "lgt_init_field")

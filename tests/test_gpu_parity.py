@@ -342,6 +342,8 @@ def test_chain_resumes_from_sweep_index(lgt, prec):
     L = (6, 4, 4, 8)
     o, U = thermal_field(L)
     with engine(lgt, L, prec, seed=21, U=U) as e:
+        assert e.extents() == (L, prec)
+        assert e.device_bytes() >= 4 * np.prod(L) * (32 if prec == 0 else 16)
         e.update(3, 1, 3.09)
         full = e.download()
         assert e.sweep_index == 3
