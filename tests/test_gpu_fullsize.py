@@ -12,6 +12,7 @@ answer.
 * Results do not depend on the tile order: sweeps, FP32 spatial smearing
   (the site-pair kernel) and the charge give the same bits under the plain
   linear order as under the tuned one.
+* The plaquette, Wilson loops and both charges are gauge invariant.
 """
 import numpy as np
 import pytest
@@ -79,3 +80,28 @@ def test_tile_order_does_not_change_results(lgt, thermal):
             res.append((e.download(), e.qtop(lgt.Q_IMPROVED)))
     np.testing.assert_array_equal(res[0][0], res[1][0])
     assert res[0][1] == res[1][1]
+
+
+def test_observables_are_gauge_invariant(lgt, thermal):
+    """The plaquette, Wilson loops and both charges of a random gauge
+    transform U'_mu(x) = g(x) U_mu(x) g(x+mu)^+ of the 48^4 field equal the
+    untransformed ones (FP64: to the rounding of the transform; FP32: to the
+    rounding of the transformed links to FP32)."""
+    import fields as F
+    prec, U = thermal
+    g = F.haar(np.random.default_rng(5), (U.shape[0],))
+    V = F.gauge_transform(U, g, L)
+    R, T = [1, 3, 5], [1, 4, 5]
+    out = []
+    for W in (U, V):
+        with lgt.Engine(L, prec, 7) as e:
+            e.upload(W)
+            out.append((e.plaquette(), e.wilson_loops(R, T), e.qtop(lgt.Q_IMPROVED), e.qtop(lgt.Q_CLOVER)))
+    (p0, w0, q0, c0), (p1, w1, q1, c1) = out
+    tol = 1e-13 if prec == 0 else 1e-8   # measured: 0 and 1.5e-11 (P), 7e-11 (W)
+    print(f"prec {prec}: dP {p1 - p0:.1e} dW {np.max(np.abs(np.subtract(w1, w0))):.1e} "
+          f"dQ {q1[0] - q0[0]:.1e} (sum |q| {q0[1]:.1f}) dQc {c1[0] - c0[0]:.1e}")
+    assert abs(p1 - p0) <= tol
+    np.testing.assert_allclose(w1, w0, rtol=0, atol=tol)
+    assert abs(q1[0] - q0[0]) <= (1e-12 if prec == 0 else 1e-8) * q0[1]   # measured 3e-19, 1e-10
+    assert abs(c1[0] - c0[0]) <= (1e-12 if prec == 0 else 1e-8) * c0[1]
