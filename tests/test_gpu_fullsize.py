@@ -14,6 +14,7 @@ answer.
   linear order as under the tuned one.
 * The plaquette, Wilson loops and both charges are gauge invariant, and APE
   smearing is gauge covariant.
+* A heatbath sub-sweep does not read the links it replaces.
 """
 import numpy as np
 import pytest
@@ -124,3 +125,23 @@ def test_smearing_is_gauge_covariant(lgt, thermal, mode, sp):
     d = np.max(np.abs(F.gauge_transform(out[0], g, L) - out[1]))
     print(f"prec {prec} mode {mode}: max |g S(U) g+ - S(g U g+)| = {d:.1e}")
     assert d <= (1e-14 if prec == 0 else 1e-6)   # measured: 8e-16, 2.5e-7
+
+
+def test_heatbath_ignores_the_links_it_replaces(lgt, thermal):
+    """A heatbath sub-sweep draws each new link from its staple and its own
+    Philox stream only: two fields that differ only in the links the sub-sweep
+    replaces give the same result bit for bit."""
+    import fields as F
+    prec, U = thermal
+    mu, p = 1, 1
+    sites = np.flatnonzero(F.coords(L).sum(axis=0) % 2 == p)
+    U2 = U.copy()
+    U2[sites, mu] = F.haar(np.random.default_rng(2), (sites.size,))
+    out = []
+    for W in (U, U2):
+        with lgt.Engine(L, prec, 13) as e:
+            e.upload(W)
+            e.heatbath_subsweep(mu, p, 3.09)
+            out.append(e.download())
+    np.testing.assert_array_equal(out[0], out[1])
+    assert np.max(np.abs(out[0][sites, mu] - U[sites, mu])) > 0.1
