@@ -248,9 +248,9 @@ def c4(quick=False):
             "seconds": time.perf_counter() - t0}
 
 
-def c6(quick=False):
+def c6(quick=False, nseeds=8):
     L = (48, 48, 48, 48)
-    seeds = (7, 8) if quick else tuple(range(7, 15))
+    seeds = (7, 8) if quick else tuple(range(7, 7 + nseeds))
     nmeas = 300 if quick else 1500
     t0 = time.perf_counter()
     rows = []
@@ -325,12 +325,13 @@ def main():
     ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
     ap.add_argument("study", choices=["c1", "c2", "c3", "c4", "c5", "c6", "all"])
     ap.add_argument("--quick", action="store_true", help="fewer sweeps (a functional check)")
+    ap.add_argument("--seeds", type=int, default=8, help="C6: independent chains per precision")
     a = ap.parse_args()
     E.load_library()
     todo = ["c1", "c2", "c3", "c4", "c5", "c6"] if a.study == "all" else [a.study]
     for s in todo:
         log(f"[{s}] start")
-        r = globals()[s](a.quick)
+        r = c6(a.quick, a.seeds) if s == "c6" else globals()[s](a.quick)
         print(json.dumps(r), flush=True)
         log(f"[{s}] done in {r['seconds']:.1f} s")
 
