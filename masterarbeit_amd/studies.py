@@ -2,7 +2,7 @@
 """Physics studies on the GPU engine: thermalisation, FP32-storage precision,
 topology and a 96^4 production chain.
 
-    python -m masterarbeit_amd.studies c1|c2|c3|c4|c5|c6|all [--quick]
+    python -m masterarbeit_amd.studies c1|c2|c3|c4|c5|c6|c7|all [--quick]
 
 Each study prints one JSON line on stdout (progress on stderr).  The field is
 resident on the device throughout; host transfers happen only where a
@@ -26,9 +26,13 @@ and FP64 storage) or restored after a destructive measurement.
   c5  96^4, FP32 storage, hot start (seed 3): 200 sweeps; 400 spatial APE
       steps and W at R = T in {5, 7, 9}; 30 steps of 4D APE and Q; device
       memory against the 288 GB of one MI355X.
-  c6  48^4: <P> of FP32-storage and FP64 chains from 8 seeds (7..14), 200
-      thermalisation + 1500 measured sweeps each; the differences against
-      their autocorrelation-corrected errors.
+  c6  48^4: <P> of FP32-storage and FP64 chains from 8 seeds (7..14, or
+      --seeds N), 200 thermalisation + 1500 measured sweeps each; the
+      differences against their autocorrelation-corrected errors.
+  c7  48^4: unsmeared W(1,1), W(5,5), W(7,7), W(9,9) of FP64 and FP32-storage
+      chains (4 seeds, hot and cold starts, 3000 thermalisation sweeps, 1000
+      measurements 4 sweeps apart) against the reference's own chain at
+      beta 3.09 (W_REF).
 """
 import argparse
 import json
@@ -280,6 +284,49 @@ def c6(quick=False, nseeds=8):
             "rms_z": float(np.sqrt(np.mean((d / sig) ** 2))), "seconds": time.perf_counter() - t0}
 
 
+# the reference's unsmeared loops at beta 3.09, 48^4: mean and autocorrelation-
+# corrected error of its configurations 200..999 (of 1000) in
+# measurements/Therm/48x48x48x48_SU2_b3.090000_id1.meas (the same tau_int
+# method as here)
+W_REF = {1: (0.732540, 3.7e-6), 5: (0.044296, 1.5e-5), 7: (0.009507, 9.3e-6), 9: (0.001973, 4.9e-6)}
+
+
+def c7(quick=False):
+    # Large loops relax slowly under local updates (W(9,9) drifts for ~2000
+    # sweeps from a hot start, from below, and from a cold start from above),
+    # so the chains start alternately hot and cold, thermalise 3000 sweeps and
+    # measure every 4th of the next 4000.
+    L = (48, 48, 48, 48)
+    seeds = (7,) if quick else (7, 8, 9, 10)
+    therm, nmeas, every = (100, 100, 1) if quick else (3000, 1000, 4)
+    Rs = sorted(W_REF)
+    t0 = time.perf_counter()
+    out = {}
+    for prec, name in ((E.FP64, "fp64"), (E.FP32, "fp32")):
+        means, errs = [], []
+        for k, seed in enumerate(seeds):
+            with E.Engine(L, prec, seed) as e:
+                e.init_field(1 - k % 2)
+                e.update(therm, 1, BETA)
+                w = []
+                for _ in range(nmeas):
+                    e.update(every, 1, BETA)
+                    w.append(e.wilson_loops(Rs, Rs))
+            w = np.array(w)
+            means.append(w.mean(axis=0))
+            errs.append([tau_int(w[:, k])[1] for k in range(len(Rs))])
+            log(f"[c7] {name} seed {seed}: W = {np.array2string(means[-1], precision=6)}")
+        m = np.mean(means, axis=0)
+        err = np.sqrt(np.sum(np.array(errs) ** 2, axis=0)) / len(seeds)
+        out[name] = {f"W{R}x{R}": {"mean": float(m[k]), "err": float(err[k]), "reference": W_REF[R][0],
+                                   "reference_err": W_REF[R][1], "diff": float(m[k] - W_REF[R][0]),
+                                   "z": float((m[k] - W_REF[R][0]) / np.hypot(err[k], W_REF[R][1]))}
+                     for k, R in enumerate(Rs)}
+    return {"study": "c7", "lattice": L, "beta": BETA, "seeds": list(seeds), "starts": "hot, cold alternately",
+            "thermalisation": therm, "measurements": nmeas, "sweeps_between": every, "loops": out,
+            "seconds": time.perf_counter() - t0}
+
+
 def c5(quick=False):
     L = (96, 96, 96, 96)
     nsweeps = 20 if quick else 200
@@ -323,12 +370,12 @@ def c5(quick=False):
 
 def main():
     ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
-    ap.add_argument("study", choices=["c1", "c2", "c3", "c4", "c5", "c6", "all"])
+    ap.add_argument("study", choices=["c1", "c2", "c3", "c4", "c5", "c6", "c7", "all"])
     ap.add_argument("--quick", action="store_true", help="fewer sweeps (a functional check)")
     ap.add_argument("--seeds", type=int, default=8, help="C6: independent chains per precision")
     a = ap.parse_args()
     E.load_library()
-    todo = ["c1", "c2", "c3", "c4", "c5", "c6"] if a.study == "all" else [a.study]
+    todo = ["c1", "c2", "c3", "c4", "c5", "c6", "c7"] if a.study == "all" else [a.study]
     for s in todo:
         log(f"[{s}] start")
         r = c6(a.quick, a.seeds) if s == "c6" else globals()[s](a.quick)
