@@ -70,6 +70,17 @@ __device__ __forceinline__ uint32_t tile_base_pair(const Geo& g, int& p)
     return tile_map(g, b >> 1);
 }
 
+// Both parities of a tile in one block (waves alternate between the parities,
+// consecutive blocks take the two halves of the tile), so that the links the
+// two parities share are read through one CU's L1.  Sets p and returns the
+// thread's cb index.
+__device__ __forceinline__ uint32_t tile_site_mixed(const Geo& g, int& p)
+{
+    const uint32_t b = xcd_remap(g, blockIdx.x, gridDim.x);
+    p = (int)((threadIdx.x >> 6) & 1u);
+    return tile_map(g, b >> 1) + (b & 1u) * 128u + ((threadIdx.x >> 7) << 6) + (threadIdx.x & 63u);
+}
+
 // keeps q's value in registers at this point of the program: what computes it
 // is not moved past the following statements
 template <typename T>
@@ -696,12 +707,16 @@ template <typename T> struct NormPlanes { T* p[4][2]; };
 
 // One thread per site.  grid: x over (tile, parity) pairs.  With Mn's planes
 // of the smeared directions set, |M| of every smeared link is stored too.
+// FP64 spatial smearing puts both parities of a tile in one block
+// (tile_site_mixed: 347 instead of 363 us at 48^4; FP32 changes by 1%, FP64
+// 4D smearing runs 31% slower with it).
 template <typename T, int MODE>
 __global__ void __launch_bounds__(BLOCK) k_ape(Planes<T> Z, Planes<T> O, Geo g, T sp, NormPlanes<T> Mn)
 {
     constexpr int MU0 = (MODE == 0) ? 1 : 0, MU1 = (MODE == 2) ? 0 : 3;
+    constexpr bool MIXED = sizeof(T) == 8 && MODE == 0;
     int p;
-    uint32_t i = tile_base_pair(g, p) + threadIdx.x;
+    uint32_t i = MIXED ? tile_site_mixed(g, p) : tile_base_pair(g, p) + threadIdx.x;
     if (i >= g.vh) return;
     const Site s = cb_site(g, i, p);
     Site spn[4], smn[4];
