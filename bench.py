@@ -36,7 +36,7 @@ HBM_PEAK_GBS = 8000.0
 # per kernel pass; heatbath reads 7 half-planes and writes 1, overrelaxation
 # also reads the old link
 BYTES_PER_UPDATE = {"heatbath": {1: 128, 0: 256}, "overrelax": {1: 144, 0: 288}}
-APE_BYTES_PER_LINK = {1: 32, 0: 64}
+LINK_BYTES = {1: 16, 0: 32}   # FP32 storage, FP64
 
 
 def log(msg):
@@ -91,11 +91,91 @@ def run_chain(lgt, L, prec, seed, beta, n_or, warmup, steps, nplaq):
     }
 
 
-def time_ape(e, prec, L, nsteps=20, mode=0, sp=0.25):
-    """One APE step per launch; algorithmic bytes: every smeared link read and
-    written once (spatial 3 VOL, 4D 4 VOL, temporal VOL links)."""
+# Compulsory link transfers of one APE step, per smeared link: spatial
+# smearing reads and writes the 3 VOL spatial links (its staples use only
+# spatial links, smearing.c:107-138), 4D smearing the 4 VOL links
+# (smearing.c:74-105); temporal smearing writes the VOL temporal links but its
+# staples read all 4 VOL links (smearing.c:140-168), i.e. 5 transfers per
+# smeared link.
+APE_SMEARED = {0: 3, 1: 4, 2: 1}
+APE_TRANSFERS = {0: 2, 1: 2, 2: 5}
+
+
+def time_heatbath_only(e, L, beta, steps):
+    """Heatbath-only sweeps, the reference's algorithm (update.c:102-176: one
+    heatbath sweep, then one projection of the field): the GPU figure that
+    stands beside the heatbath-only CPU baseline."""
     vol = int(np.prod(L))
-    nlinks = {0: 3, 1: 4, 2: 1}[mode] * vol
+    e.sync()
+    t0 = time.perf_counter()
+    e.update(steps, 0, beta)
+    e.sync()
+    dt = time.perf_counter() - t0
+    return {"link_updates_per_s": 4.0 * vol * steps / dt, "ms_per_step": 1e3 * dt / steps, "steps": steps}
+
+
+def cpu_model():
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return None
+
+
+def observables_24(lgt, beta, seed=5, reps=5):
+    """One spatial APE step (sp = 0.25), the Wilson loops R,T in 1..6 and the
+    improved charge at 24^4 in FP64: the reference's own functions
+    (oracle/_ref, one host core) and the engine on the same configuration,
+    timed in the same run."""
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    from oracle_lib import RefLib
+    if not RefLib.available("l24"):
+        return None
+    L = (24, 24, 24, 24)
+    R6, T6 = np.meshgrid(np.arange(1, 7), np.arange(1, 7), indexing="ij")
+    R6, T6 = R6.ravel().astype(np.int32), T6.ravel().astype(np.int32)
+
+    def timed(f, n=1):
+        t0 = time.perf_counter()
+        for _ in range(n):
+            f()
+        return (time.perf_counter() - t0) / n
+
+    with lgt.Engine(L, lgt.FP64, seed) as e:
+        e.init_field(1)
+        e.update(50, 1, beta)
+        U = e.download()
+
+        def gpu(f):
+            f()
+            e.sync()
+
+        gpu_ape = timed(lambda: gpu(lambda: e.ape(0, 1, 0.25)), reps)
+        e.upload(U)
+        gpu_w = timed(lambda: e.wilson_loops(R6, T6), reps)
+        gpu_q = timed(lambda: e.qtop(0), reps)
+    ref = RefLib("l24")
+    ref.setup(beta, 1)
+    ref.set_field(U)
+    cpu_ape = timed(lambda: ref.lib.ref_ape(0, 1, 0.25))
+    ref.set_field(U)
+    cpu_w = timed(lambda: ref.wilson(R6, T6))
+    cpu_q = timed(lambda: ref.lib.ref_qtop_improved())
+    rows = []
+    for op, c, g in (("spatial APE step", cpu_ape, gpu_ape), ("Wilson loops R,T 1..6", cpu_w, gpu_w),
+                     ("improved charge", cpu_q, gpu_q)):
+        rows.append({"op": op, "reference_cpu_s": c, "gpu_fp64_s": g, "speedup": c / g})
+    return {"lattice": list(L), "cores": 1, "cpu_model": cpu_model(), "rows": rows}
+
+
+def time_ape(e, prec, L, nsteps=20, mode=0, sp=0.25):
+    """One APE step per launch; algorithmic bytes: APE_TRANSFERS link
+    transfers per smeared link (links of 16 or 32 bytes)."""
+    vol = int(np.prod(L))
+    nlinks = APE_SMEARED[mode] * vol
     e.sync()
     t0 = time.perf_counter()
     e.ape(mode, nsteps, sp)
@@ -107,8 +187,10 @@ def time_ape(e, prec, L, nsteps=20, mode=0, sp=0.25):
     ms, n = e.profile_read(4)
     e.set_option(1, 0)
     avg = ms / n
-    gbs = nlinks * APE_BYTES_PER_LINK[prec] / (avg * 1e-3) / 1e9
-    return {"steps_per_s": nsteps / dt, "kernel_ms_avg": avg, "achieved_GBs": gbs, "frac_hbm": gbs / HBM_PEAK_GBS}
+    nbytes = nlinks * APE_TRANSFERS[mode] * LINK_BYTES[prec]
+    gbs = nbytes / (avg * 1e-3) / 1e9
+    return {"steps_per_s": nsteps / dt, "kernel_ms_avg": avg, "algorithmic_bytes": nbytes, "achieved_GBs": gbs,
+            "frac_hbm": gbs / HBM_PEAK_GBS}
 
 
 def time_plaquette(e, prec, L, reps=20):
@@ -121,7 +203,7 @@ def time_plaquette(e, prec, L, reps=20):
     ms, n = e.profile_read(3)
     e.set_option(1, 0)
     avg = ms / n
-    gbs = vol * 4 * APE_BYTES_PER_LINK[prec] / 2 / (avg * 1e-3) / 1e9
+    gbs = vol * 4 * LINK_BYTES[prec] / (avg * 1e-3) / 1e9
     return {"kernel_ms_avg": avg, "achieved_GBs": gbs, "frac_hbm": gbs / HBM_PEAK_GBS}
 
 
@@ -188,9 +270,12 @@ FLOP_QTOP_SITE = 16100
 
 def time_observables(e, L):
     """Wilson loops R 1..10 x T 1..16 and the improved charge, with their rate
-    in the reference's operation count: a loop costs 3 directions x 2(R+T)
-    products per site (wilson.c:21-56), the charge ~16 100 flops per site
-    (topologicalcharge.c:112-199); the fused loop kernel performs fewer."""
+    in reference-equivalent operations: the flops the reference's algorithm
+    would spend (a loop costs 3 directions x 2(R+T) products per site,
+    wilson.c:21-56; the charge ~16 100 flops per site,
+    topologicalcharge.c:112-199) divided by the GPU time.  The kernels here
+    perform far fewer (shared line products), so this is not an executed
+    FLOP rate and is not comparable with a FLOP/s peak."""
     vol = int(np.prod(L))
     R, T = np.meshgrid(np.arange(1, 11), np.arange(1, 17), indexing="ij")
     R, T = R.ravel(), T.ravel()
@@ -204,8 +289,8 @@ def time_observables(e, L):
     fw = float(vol * 3 * np.sum(2 * (R + T)) * FLOP_QMUL)
     fq = float(vol * FLOP_QTOP_SITE)
     return {
-        "wloops_R1-10_T1-16": {"s": tw, "reference_flops": fw, "TFLOPs": fw / tw / 1e12},
-        "qtop_improved": {"s": tq, "reference_flops": fq, "TFLOPs": fq / tq / 1e12},
+        "wloops_R1-10_T1-16": {"s": tw, "reference_flops": fw, "ref_equiv_Tops_per_s": fw / tw / 1e12},
+        "qtop_improved": {"s": tq, "reference_flops": fq, "ref_equiv_Tops_per_s": fq / tq / 1e12},
     }
 
 
@@ -383,6 +468,8 @@ def main():
 
     e32, r32 = run_chain(lgt, L, lgt.FP32, args.seed, args.beta, args.n_or, args.warmup, args.steps, args.nplaq)
     log(f"[bench] fp32 sweep: {r32['ms_per_step']:.3f} ms/step, {r32['link_updates_per_s']:.4g} link updates/s")
+    hb32 = time_heatbath_only(e32, L, args.beta, args.steps)
+    log(f"[bench] fp32 heatbath-only: {hb32['link_updates_per_s']:.4g} link updates/s")
     stream32 = time_stream(e32, 1, L)
     log(f"[bench] fp32 streaming pass (project): {stream32['achieved_GBs']:.0f} GB/s")
     plaq32 = time_plaquette(e32, 1, L)
@@ -396,6 +483,8 @@ def main():
     e64, r64 = run_chain(lgt, L, lgt.FP64, args.seed, args.beta, args.n_or, args.warmup, args.steps, args.nplaq)
     log(f"[bench] fp64 sweep: {r64['ms_per_step']:.3f} ms/step, {r64['link_updates_per_s']:.4g} link updates/s")
     U64 = e64.download()   # the thermalised configuration: input of the CPU baseline
+    hb64 = time_heatbath_only(e64, L, args.beta, args.steps)
+    e64.upload(U64)
     plaq64 = time_plaquette(e64, 0, L)
     ape64 = time_ape(e64, 0, L)
     ape4d64 = time_ape(e64, 0, L, mode=1, sp=1.0 / 6.0)
@@ -437,8 +526,14 @@ def main():
     if not args.no_cpu_baseline:
         log("[bench] timing the reference CPU sweep (one core, then all cores) ...")
         base = cpu_baseline(U64, L, args.beta, args.cpu_procs)
+        if base is not None:
+            base["cpu_model"] = cpu_model()
     del U64
     e64.close()
+    obs24 = None
+    if not args.no_cpu_baseline:
+        log("[bench] 24^4 observables: reference CPU functions and the engine ...")
+        obs24 = observables_24(lgt, args.beta)
 
     out = {
         "metric": METRIC,
@@ -480,6 +575,10 @@ def main():
             "qcd_loop_ms_per_step": r64["qcd_loop_ms_per_step"],
             "kernels": r64["kernels"],
         },
+        "heatbath_only": {"fp32": hb32, "fp64": hb64,
+                          "note": "one heatbath sweep + one projection pass per step, the reference's algorithm; "
+                                  "compare with cpu_baseline"},
+        "observables_24_cpu_vs_gpu": obs24,
         "fp32_qcd_loop_ms_per_step": r32["qcd_loop_ms_per_step"],
         "fp32_over_fp64": r32["link_updates_per_s"] / r64["link_updates_per_s"],
         "fp32_kernels": r32["kernels"],

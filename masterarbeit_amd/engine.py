@@ -241,8 +241,10 @@ class Engine:
         _check(self._lib.lgt_ape(self._h, mode, nsteps, sp), "lgt_ape")
 
     def wilson_loops(self, R, T):
-        R = np.ascontiguousarray(R, dtype=np.int32)
-        T = np.ascontiguousarray(T, dtype=np.int32)
+        R = np.ascontiguousarray(R, dtype=np.int32).ravel()
+        T = np.ascontiguousarray(T, dtype=np.int32).ravel()
+        if len(R) != len(T):
+            raise ValueError(f"wilson_loops: {len(R)} spatial extents but {len(T)} temporal extents")
         w = np.zeros(len(R))
         _check(self._lib.lgt_wilson_loops(self._h, len(R), _ptr(R, C.c_int), _ptr(T, C.c_int), _ptr(w)),
                "lgt_wilson_loops")
@@ -263,6 +265,8 @@ class Engine:
 
     def ape_vjp_backward(self, cotangent=None, grad=True):
         cot = None if cotangent is None else np.ascontiguousarray(cotangent, dtype=np.float64)
+        if cot is not None and cot.size != 16 * self.vol:
+            raise ValueError(f"ape_vjp_backward: cotangent has {cot.size} values, the field {16 * self.vol}")
         G = np.empty((self.vol, 4, 4), dtype=np.float64) if grad else None
         _check(self._lib.lgt_ape_vjp_backward(self._h, _ptr(cot) if cot is not None else None,
                                               _ptr(G) if grad else None), "lgt_ape_vjp_backward")
@@ -283,10 +287,14 @@ class Engine:
 def heatbath_harness(staples, beta, uniforms, offsets, lens):
     lib = load_library()
     staples = np.ascontiguousarray(staples, dtype=np.float64)
-    uniforms = np.ascontiguousarray(uniforms, dtype=np.float64)
-    offsets = np.ascontiguousarray(offsets, dtype=np.int32)
-    lens = np.ascontiguousarray(lens, dtype=np.int32)
+    uniforms = np.ascontiguousarray(uniforms, dtype=np.float64).ravel()
+    offsets = np.ascontiguousarray(offsets, dtype=np.int32).ravel()
+    lens = np.ascontiguousarray(lens, dtype=np.int32).ravel()
     n = staples.shape[0]
+    if staples.shape != (n, 4) or offsets.size != n or lens.size != n:
+        raise ValueError("heatbath_harness: shapes")
+    if n and (offsets.min() < 0 or lens.min() < 0 or (offsets.astype(np.int64) + lens).max() > uniforms.size):
+        raise ValueError("heatbath_harness: uniform ranges outside the uniform array")
     links = np.zeros((n, 4))
     counts = np.zeros(n, dtype=np.int32)
     _check(lib.lgt_heatbath_harness(n, _ptr(staples), beta, _ptr(uniforms), _ptr(offsets, C.c_int),
@@ -299,6 +307,8 @@ def overrelax_harness(staples, links_in):
     staples = np.ascontiguousarray(staples, dtype=np.float64)
     links_in = np.ascontiguousarray(links_in, dtype=np.float64)
     n = staples.shape[0]
+    if staples.shape != (n, 4) or links_in.shape != (n, 4):
+        raise ValueError("overrelax_harness: shapes")
     out = np.zeros((n, 4))
     _check(lib.lgt_overrelax_harness(n, _ptr(staples), _ptr(links_in), _ptr(out)), "lgt_overrelax_harness")
     return out

@@ -3,11 +3,11 @@ engine, driven by a reference-style C host program (tests/compat/
 compat_driver.c) that owns pu/latParams/runParams/measParams exactly as
 LGT_Code/main/qcd.c does, compared with the FP64 oracle."""
 import os
-import subprocess
 
 import numpy as np
 import pytest
 
+import reflogs
 from oracle_lib import Oracle
 
 pytestmark = pytest.mark.gpu
@@ -47,8 +47,8 @@ def test_reference_entry_points_through_compat(tmp_path, prec, sync, order):
     o.engine_sweep(U, 3.09, 1, 21, 0)
     U.tofile(tmp_path / "field.bin")
     env = dict(os.environ, LGT_N_OR="1", LGT_PRECISION=prec, COMPAT_SITE_ORDER=order)
-    r = subprocess.run([driver, *map(str, L), str(tmp_path / "field.bin"), str(tmp_path)], env=env,
-                       capture_output=True, text=True, timeout=120)
+    r = reflogs.run_program([driver, *map(str, L), str(tmp_path / "field.bin"), str(tmp_path)], env=env,
+                            timeout=120)
     assert r.returncode == 0, r.stderr
     s = read_summary(tmp_path / "summary.txt")
     vol = o.vol
@@ -89,7 +89,7 @@ def test_reference_entry_points_through_compat(tmp_path, prec, sync, order):
         assert (s["uploads"], s["downloads"]) == (2, 5)
     else:
         assert (s["uploads"], s["downloads"]) == (11, 5)
-    log = open(tmp_path / "log.txt").read()
+    log = reflogs.read_log(tmp_path / "log.txt")
     assert "Update 1 done:\n(algorithm 1; sweep 0; acc=" in log and "update took" in log
     assert "calculating topological charge" in log
 
@@ -113,14 +113,15 @@ def test_reference_qcd_program_on_the_engine(tmp_path, prec):
         f"outFileId     1\noutputDir     {out}\nstart         1\nbeta          3.09\nepsilon       0.15\n"
         "nconf         12\nntherm        4\nndecorr       1\nwriteConfFreq 4\nseed          1337\n")
     env = dict(os.environ, LGT_N_OR="1", LGT_PRECISION=prec)
-    r = subprocess.run([QCD, "-i", str(tmp_path / "run.in")], cwd=tmp_path, env=env, capture_output=True, text=True,
-                       timeout=120)
+    r = reflogs.run_program([QCD, "-i", str(tmp_path / "run.in")], cwd=tmp_path, env=env, timeout=120)
     assert r.returncode == 0, r.stdout[-2000:] + r.stderr[-2000:]
     logs = list(out.glob("*.log"))
     assert len(logs) == 1
-    log = logs[0].read_text()
+    # the startup block holds qcd.c's uninitialised cnfg_file (qcd.c:31,42,
+    # inp_IO.c:244): arbitrary bytes, hence the byte-safe reader
+    log = reflogs.read_log(logs[0])
     assert log.count("(algorithm 1; sweep 0; acc=") == 12
-    plaq = [float(l.split("\t")[1]) for l in log.splitlines() if l.startswith("plaq\t")]
+    plaq = reflogs.plaquettes(log)
     assert len(plaq) == 12 and plaq[0] < plaq[-1] and 0.65 < plaq[-1] < 0.8
     confs = sorted(out.glob("*_n*"))
     assert [c.name.rsplit("_", 1)[1] for c in confs] == ["n1", "n2"]
@@ -153,10 +154,9 @@ def test_reference_qtop_program_on_the_engine(tmp_path):
         e.update(30, 1, 2.3)
         U = e.download()
         e.write_config(tmp_path / "8x8x8x8_SU2_b2.300000_id1_n1")
-    r = subprocess.run([QTOP, str(tmp_path)], cwd=tmp_path, capture_output=True, text=True, timeout=120)
+    r = reflogs.run_program([QTOP, str(tmp_path)], cwd=tmp_path, timeout=120)
     assert r.returncode == 0, r.stdout[-2000:] + r.stderr[-2000:]
-    meas = (tmp_path / "8x8x8x8_SU2_b2.300000_id1.meas").read_text()
-    got = [(int(l.split()[2]), float(l.split()[4])) for l in meas.splitlines() if l.startswith("QTOP ")]
+    got = reflogs.qtop_scan(reflogs.read_log(tmp_path / "8x8x8x8_SU2_b2.300000_id1.meas"))
     assert [n for n, _ in got] == list(range(21))
     o = Oracle(L)
     V = U.copy()
@@ -193,12 +193,11 @@ def test_reference_meas_program_on_the_engine(tmp_path):
         "tLoopExtent    1 4 1\ntSmear         0 0.0\nrLoopExtent    1 3 1\nrSmear         5 0.25\n"
         "mcorrs         0\nkappa          0.12\nsource         0 0 0 0\nstype          0\nnmax           100\n"
         "prec           1e-8\nn2pt           0\n")
-    r = subprocess.run([MEAS, "-i", str(tmp_path / "meas.in")], cwd=tmp_path, capture_output=True, text=True,
-                       timeout=120)
+    r = reflogs.run_program([MEAS, "-i", str(tmp_path / "meas.in")], cwd=tmp_path, timeout=120)
     assert r.returncode == 0, r.stdout[-2000:] + r.stderr[-2000:]
-    meas = (tmp_path / "8x8x8x8_SU2_b2.300000_id1.meas").read_text().splitlines()
-    qs = [float(l.split()[1]) for l in meas if l.startswith("QTOP ")]
-    ws = [(int(l.split()[1]), int(l.split()[2]), float(l.split()[3])) for l in meas if l.startswith("WIL ")]
+    meas = reflogs.read_log(tmp_path / "8x8x8x8_SU2_b2.300000_id1.meas")
+    qs = reflogs.meas_charges(meas)
+    ws = reflogs.meas_loops(meas)
     assert len(qs) == 2 and len(ws) == 2 * 12
     o = Oracle(L)
     R = [r_ for r_ in (1, 2, 3) for _ in (1, 2, 3, 4)]
