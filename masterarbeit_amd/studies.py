@@ -327,6 +327,72 @@ def c7(quick=False):
             "seconds": time.perf_counter() - t0}
 
 
+# The reference's own 24^4 chains at beta 3.09 (heatbath, 50 000 stored
+# configurations each from a cold and a hot start): mean over configurations
+# 1000..49999 and error of the mean from the means of 49 blocks of 1000
+# configurations, cold and hot combined; Thermalisation/
+# 24x24x24x24_b3.09_WL_<R>_<T>_{cold,hot}.dat, and W(10,16) after 100 spatial
+# APE steps on configurations 1000..9999 (..._WL_10_16_{cold,hot}_NAPE_100.dat;
+# its smearing parameter is not archived with it).  The smeared series of the
+# two starts differ by 1.4e-4, five times their block error (the hot chain's
+# first 10 000 configurations have 2 tau_int = 28), so its error here is half
+# that difference.  The same figures with autocorrelation-time errors per
+# start are in tests/golden/ref_therm24.json.
+W_REF24 = {(1, 1): (0.7325384, 1.3e-6), (3, 3): (0.1955563, 6.7e-6), (5, 5): (0.0442735, 7.9e-6),
+           (7, 7): (0.0095009, 5.9e-6), (9, 9): (0.0019756, 3.6e-6), (10, 16): (8.3275e-05, 1.9e-6)}
+W_REF24_APE100 = (0.0017639, 7.0e-5)
+
+
+def c8(quick=False, nmeas=5000, every=2, therm=2000, ape_every=10):
+    """24^4 chains (FP64 and FP32 storage, cold and hot starts, heatbath + 1
+    overrelaxation per sweep): W(R,T) for the reference's six loops after
+    every `every` sweeps, and W(10,16) after 100 spatial APE steps (sp 0.25)
+    on every `ape_every`-th measured configuration, against W_REF24."""
+    L = (24, 24, 24, 24)
+    if quick:
+        nmeas, therm = 300, 200
+    keys = sorted(W_REF24)
+    R = [k[0] for k in keys]
+    T = [k[1] for k in keys]
+    t0 = time.perf_counter()
+    out = {}
+    for prec, name in ((E.FP64, "fp64"), (E.FP32, "fp32")):
+        chains = {}
+        for start, sname, seed in ((0, "cold", 21), (1, "hot", 22)):
+            t1 = time.perf_counter()
+            with E.Engine(L, prec, seed) as e, E.Engine(L, prec, seed) as m:
+                e.init_field(start)
+                e.update(therm, 1, BETA)
+                w, wa = [], []
+                for i in range(nmeas):
+                    e.update(every, 1, BETA)
+                    w.append(e.wilson_loops(R, T))
+                    if i % ape_every == 0:
+                        m.upload(e.download())
+                        m.ape(E.APE_SPATIAL, 100, SP_SPATIAL)
+                        wa.append(m.wilson_loops([10], [16])[0])
+                    if (i + 1) % 1000 == 0:
+                        log(f"[c8] {name} {sname}: {i + 1} measurements, {time.perf_counter() - t1:.1f} s")
+            w = np.array(w)
+            chains[sname] = {"w": w, "wa": np.array(wa)}
+            log(f"[c8] {name} {sname}: {time.perf_counter() - t1:.1f} s, W = {np.array2string(w.mean(axis=0), precision=7)}"
+                f", APE-100 W(10,16) = {np.mean(wa):.6g}")
+        res = {}
+        for k, key in enumerate(keys + [("ape100", 10, 16)]):
+            series = [c["wa"] if key[0] == "ape100" else c["w"][:, k] for c in chains.values()]
+            stats = [tau_int(x) for x in series]
+            m_ = float(np.mean([x.mean() for x in series]))
+            err = float(np.sqrt(sum(s_[1] ** 2 for s_ in stats)) / len(series))
+            ref, ref_err = W_REF24_APE100 if key[0] == "ape100" else W_REF24[key]
+            label = "APE100 W10x16" if key[0] == "ape100" else f"W{key[0]}x{key[1]}"
+            res[label] = {"mean": m_, "err": err, "tau_int": [s_[0] for s_ in stats],
+                          "per_start": [float(x.mean()) for x in series], "reference": ref,
+                          "reference_err": ref_err, "z": (m_ - ref) / float(np.hypot(err, ref_err))}
+        out[name] = res
+    return {"study": "c8", "lattice": L, "beta": BETA, "thermalisation": therm, "measurements": nmeas,
+            "sweeps_between": every, "ape_every": ape_every, "loops": out, "seconds": time.perf_counter() - t0}
+
+
 def c5(quick=False):
     L = (96, 96, 96, 96)
     nsweeps = 20 if quick else 200
@@ -370,12 +436,12 @@ def c5(quick=False):
 
 def main():
     ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
-    ap.add_argument("study", choices=["c1", "c2", "c3", "c4", "c5", "c6", "c7", "all"])
+    ap.add_argument("study", choices=["c1", "c2", "c3", "c4", "c5", "c6", "c7", "c8", "all"])
     ap.add_argument("--quick", action="store_true", help="fewer sweeps (a functional check)")
     ap.add_argument("--seeds", type=int, default=8, help="C6: independent chains per precision")
     a = ap.parse_args()
     E.load_library()
-    todo = ["c1", "c2", "c3", "c4", "c5", "c6", "c7"] if a.study == "all" else [a.study]
+    todo = ["c1", "c2", "c3", "c4", "c5", "c6", "c7", "c8"] if a.study == "all" else [a.study]
     for s in todo:
         log(f"[{s}] start")
         r = c6(a.quick, a.seeds) if s == "c6" else globals()[s](a.quick)
