@@ -88,9 +88,18 @@ extern void ranlxd(double r[], int n);                    /* ranlxd.c:571-586 */
 /* ---- exported entry points (LGT_Code/include/modules.h) ---- */
 /* modules.h:99, update.c:102-176: utype 1 = heatbath (+ LGT_N_OR overrelaxation
  * sweeps), 0 = Metropolis with runParams.eps; the GPU sweep is checkerboard
- * whatever stype says. */
+ * whatever stype says (stype 1, random site order, logs a warning once). */
 void gaugefieldUpdate(int iup, int nup, int utype, int stype);
-double plaquette(void);                                   /* modules.h:27, plaquette.c:67-101 */
+/* modules.h:89, metro.c:34-63: the staple of pu's link (n, dir), computed on the device */
+void staples(uint64_t n, int dir, su2mat *stap);
+/* modules.h:95, heatbath.c:124-149: one link's heatbath on the device with
+ * uniforms drawn from the program's ranlxd in the reference's pattern; the new
+ * link goes to the device field and to pu[n][dir].  One device round trip per
+ * trial: for whole sweeps gaugefieldUpdate is the fast path. */
+double localHeatbathUpdate(uint64_t n, int dir, int m);
+/* modules.h:27, plaquette.c:67-101; when pu is uploaded by the call, the FP64
+ * plaquette of pu's links as given (so readConfig's test holds in FP32 storage) */
+double plaquette(void);
 double gaugeAction(void);                                 /* modules.h:28, plaquette.c:104-110 */
 void smearing_APE_all(int ns, double sp, su2mat *u[][4]);      /* modules.h:32, smearing.c:74-105 */
 void smearing_APE_spatial(int ns, double sp, su2mat *u[][4]);  /* modules.h:33, smearing.c:107-138 */

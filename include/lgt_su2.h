@@ -115,6 +115,11 @@ int lgt_init_field(lgt_engine *e, int start);
  * between file and pu. */
 int lgt_upload(lgt_engine *e, const double *aos);
 int lgt_download(lgt_engine *e, double *aos);
+/* lgt_upload, and plaq = the plaquette of the links as given (FP64, before an
+ * FP32 engine rounds them): the value readConfig tests against the file's
+ * (LGT_Code/modules/io/config_IO.c:609-614, plaquette.c:67-101), which an
+ * FP32 engine's lgt_plaquette of the rounded links would miss by ~1e-10. */
+int lgt_upload_plaquette(lgt_engine *e, const double *aos, double *plaq);
 
 /* writeConfig / readConfig (LGT_Code/modules/io/config_IO.c:40-117,537-621):
  * the reference's binary configuration file, little-endian: int32 DIM = 4,
@@ -122,11 +127,11 @@ int lgt_download(lgt_engine *e, double *aos);
  * site (z fastest) and direction the link's c0..c3 as doubles.
  * lgt_write_config stores the engine's field with its plaquette.
  * lgt_read_config checks DIM, SUN and the extents (LGT_EINVAL otherwise),
- * loads the field and recomputes the plaquette; it returns LGT_EINVAL ("plaquette
- * test failed", the field stays loaded) when it differs from the file's by
- * more than sqrt(6 VOL) eps, eps = DBL_EPSILON (FP64 engine, the reference's
- * test) or FLT_EPSILON (FP32 storage).  plaq_file / plaq_calc (nullable)
- * return the two values. */
+ * loads the field and recomputes the plaquette of the links as read (FP64,
+ * also in FP32 engines, see lgt_upload_plaquette); it returns LGT_EINVAL
+ * ("plaquette test failed", the field stays loaded) when it differs from the
+ * file's by more than sqrt(6 VOL) DBL_EPSILON, the reference's test.
+ * plaq_file / plaq_calc (nullable) return the two values. */
 int lgt_write_config(lgt_engine *e, const char *path);
 int lgt_read_config(lgt_engine *e, const char *path, double *plaq_file, double *plaq_calc);
 /* readConfigForThermalisation (LGT_Code/modules/io/config_IO.c:348-535): loads
@@ -163,6 +168,28 @@ int lgt_heatbath_subsweep(lgt_engine *e, int mu, int parity, double beta, double
 int lgt_overrelax_subsweep(lgt_engine *e, int mu, int parity);
 /* project_gfield_to_sun (LGT_Code/modules/admin/sun_vfunc.c:82-97) */
 int lgt_project(lgt_engine *e);
+
+/* ---- single links, for programs that update link by link ----------------
+ * site is the lexicographic index n = x3 + L3*(x2 + L2*(x1 + L1*x0)).
+ * staples(n, dir, &A) (LGT_Code/modules/update/metro.c:34-63): A = the
+ * staple of link (site, mu) of the current field, in the reference's term
+ * order (FP64 engines bit-identical; FP32 engines widen their FP32 sum). */
+int lgt_staple(lgt_engine *e, int64_t site, int mu, double A[4]);
+/* localHeatbathUpdate(n, dir, m) (LGT_Code/modules/update/heatbath.c:124-149)
+ * fed with injected uniforms: the staple of link (site, mu), then the
+ * Kennedy-Pendleton draw (FP64, heatbath.c:76-121) or, when |A| <=
+ * DBL_EPSILON, su2RandomMatrix (random_su3.c:141-155), consuming
+ * uniforms[0 .. n_uniforms) in the reference's order (3 on the degenerate
+ * branch; else 4 per trial, then 2).  If they suffice, the new link
+ * U = X V^+ replaces the link on the device (rounded to the storage
+ * precision), link (nullable) receives it, *trials = the trial count (0: the
+ * degenerate branch) and *needed = the uniforms consumed.  If they run out,
+ * nothing changes, *trials = -1 and *needed = how many the next call needs
+ * at least (those read so far plus the next trial, direction or degenerate
+ * link): a caller that draws exactly that many more from its stream
+ * reproduces the reference's draws. */
+int lgt_local_heatbath(lgt_engine *e, int64_t site, int mu, double beta, const double *uniforms, int n_uniforms,
+                       double link[4], int *trials, int *needed);
 /* sweep counter of the RNG (advanced by lgt_update; set to resume a chain) */
 int lgt_set_sweep_index(lgt_engine *e, uint32_t sweep);
 uint32_t lgt_get_sweep_index(const lgt_engine *e);

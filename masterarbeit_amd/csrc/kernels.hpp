@@ -276,24 +276,28 @@ __device__ __forceinline__ Q<T> update_staple(const Planes<T>& P, const Geo& g, 
 // rejections (only for alpha -> 0) end it with accepted = false.
 constexpr int KP_MAX_TRIALS = 10000;
 
+// one trial of the rejection loop (heatbath.c:84-95): 4 uniforms; returns
+// whether it is accepted, with its delta
 template <class Src>
-__device__ __forceinline__ int kp_draw64(double alpha, Src& src, Q<double>& X, bool& accepted)
+__device__ __forceinline__ bool kp_trial64(double alpha, Src& src, double& delta)
 {
-    int count = 0;
-    double delta, u3;
-    do {
-        double u0 = src.next(), u1 = src.next(), u2 = src.next();
-        u3 = src.next();
-        double X1 = -log(1.0 - u0) / alpha;
-        double X2 = -log(1.0 - u1) / alpha;
-        double cc = cos(2.0 * PI_D * u2);
-        double C = cc * cc;
-        double A = X1 * C;
-        delta = X2 + A;
-        count++;
-        accepted = !(u3 * u3 > 1.0 - (0.5 * delta));
-    } while (!accepted && count < KP_MAX_TRIALS && !src.dry());
-    if (!accepted) return count;
+    double u0 = src.next(), u1 = src.next(), u2 = src.next();
+    double u3 = src.next();
+    double X1 = -log(1.0 - u0) / alpha;
+    double X2 = -log(1.0 - u1) / alpha;
+    double cc = cos(2.0 * PI_D * u2);
+    double C = cc * cc;
+    double A = X1 * C;
+    delta = X2 + A;
+    return !(u3 * u3 > 1.0 - (0.5 * delta));
+}
+
+// the accepted x0 = 1 - delta and its direction on S^2 (heatbath.c:97-107):
+// 2 uniforms
+template <class Src>
+__device__ __forceinline__ Q<double> kp_finish64(double delta, Src& src)
+{
+    Q<double> X;
     double x0 = 1.0 - delta;
     double a0 = src.next(), a1 = src.next();
     double phi = 2.0 * PI_D * a0;
@@ -303,6 +307,20 @@ __device__ __forceinline__ int kp_draw64(double alpha, Src& src, Q<double>& X, b
     X.c1 = length * sin(theta) * cos(phi);
     X.c2 = length * sin(theta) * sin(phi);
     X.c3 = length * cos(theta);
+    return X;
+}
+
+template <class Src>
+__device__ __forceinline__ int kp_draw64(double alpha, Src& src, Q<double>& X, bool& accepted)
+{
+    int count = 0;
+    double delta;
+    do {
+        accepted = kp_trial64(alpha, src, delta);
+        count++;
+    } while (!accepted && count < KP_MAX_TRIALS && !src.dry());
+    if (!accepted) return count;
+    X = kp_finish64(delta, src);
     return count;
 }
 
@@ -608,11 +626,110 @@ static __global__ void __launch_bounds__(BLOCK) k_or_harness(int n, const double
     links[4 * i] = R.c0; links[4 * i + 1] = R.c1; links[4 * i + 2] = R.c2; links[4 * i + 3] = R.c3;
 }
 
+template <typename T>
+__device__ __forceinline__ Q<double> widen(const Q<T>& q) { return Q<double>{q.c0, q.c1, q.c2, q.c3}; }
+
+// ---------------------------------------------------------------------------
+// one link of the device field: staples(n, dir, &A) (LGT_Code/modules/update/
+// metro.c:34-63) and localHeatbathUpdate(n, dir, m) (heatbath.c:124-149) at a
+// lexicographic site n, for programs that call them link by link.  One
+// thread; FP32 fields compute the staple in FP32 and run the FP64 sampler.
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ Site lex_site(const Geo& g, uint32_t n)
+{
+    Site s;
+    s.c[3] = (int)(n % (uint32_t)g.L[3]);
+    n /= (uint32_t)g.L[3];
+    s.c[2] = (int)(n % (uint32_t)g.L[2]);
+    n /= (uint32_t)g.L[2];
+    s.c[1] = (int)(n % (uint32_t)g.L[1]);
+    s.c[0] = (int)(n / (uint32_t)g.L[1]);
+    return s;
+}
+
+template <typename T>
+__device__ __forceinline__ Q<T> update_staple_mu(const Planes<T>& P, const Geo& g, const Site& s, int p, int mu)
+{
+    switch (mu) {
+    case 0: return update_staple<T, 0>(P, g, s, p);
+    case 1: return update_staple<T, 1>(P, g, s, p);
+    case 2: return update_staple<T, 2>(P, g, s, p);
+    default: return update_staple<T, 3>(P, g, s, p);
+    }
+}
+
+// out[0..3] = the staple of link (n, mu)
+template <typename T>
+__global__ void __launch_bounds__(64) k_staple_link(Planes<T> P, Geo g, uint32_t n, int mu, double* __restrict__ out)
+{
+    if (threadIdx.x != 0) return;
+    const Site s = lex_site(g, n);
+    const Q<double> A = widen(update_staple_mu(P, g, s, parity(s), mu));
+    out[0] = A.c0; out[1] = A.c1; out[2] = A.c2; out[3] = A.c3;
+}
+
+// The heatbath of link (n, mu) fed with the uniforms u[0 .. nu), consumed in
+// the reference's order (3 on the degenerate branch; else 4 per trial, then
+// 2).  When they last: the new link is stored and also written to link_out,
+// status[0] = trials (0: degenerate branch), status[1] = uniforms consumed.
+// When they run out: nothing is stored, status[0] = -1 and status[1] = the
+// number of uniforms the next attempt needs at least (the uniforms read so
+// far plus those of the next step: a trial, the direction or the degenerate
+// link), so a caller can draw exactly what the reference would draw.
+template <typename T>
+__global__ void __launch_bounds__(64) k_local_heatbath(Planes<T> P, Geo g, uint32_t n, int mu, double beta,
+                                                        const double* __restrict__ u, int nu,
+                                                        double* __restrict__ link_out, int* __restrict__ status)
+{
+    if (threadIdx.x != 0) return;
+    const Site s = lex_site(g, n);
+    const int p = parity(s);
+    Q<double> A = widen(update_staple_mu(P, g, s, p, mu));
+    BufSrc src{u, nu, 0};
+    Q<double> Un;
+    int trials = 0;
+    double sqrt_det = qdet_sqrt(A);
+    if (sqrt_det <= __DBL_EPSILON__) {
+        if (nu < 3) {
+            status[0] = -1;
+            status[1] = 3;
+            return;
+        }
+        Un = random_su2_64(src);
+    } else {
+        A = qdiv(A, sqrt_det);
+        const double alpha = sqrt_det * beta;
+        double delta = 0.0;
+        bool accepted = false;
+        while (!accepted) {
+            if (trials == KP_MAX_TRIALS) break;
+            if (src.pos + 4 > nu) {
+                status[0] = -1;
+                status[1] = src.pos + 4;
+                return;
+            }
+            accepted = kp_trial64(alpha, src, delta);
+            trials++;
+        }
+        const int rest = accepted ? 2 : 3;   // the direction, or a uniform link after KP_MAX_TRIALS
+        if (src.pos + rest > nu) {
+            status[0] = -1;
+            status[1] = src.pos + rest;
+            return;
+        }
+        Un = accepted ? qmul_dag(kp_finish64(delta, src), A) : random_su2_64(src);
+    }
+    const Q<T> st{(T)Un.c0, (T)Un.c1, (T)Un.c2, (T)Un.c3};
+    qstore<T>(P.p[mu][p], cb_index(g, s), st);
+    const Q<double> w = widen(st);
+    link_out[0] = w.c0; link_out[1] = w.c1; link_out[2] = w.c2; link_out[3] = w.c3;
+    status[0] = trials;
+    status[1] = src.pos;
+}
+
 // ---------------------------------------------------------------------------
 // plaquette (LGT_Code/modules/obs/plaquette.c:67-101)
 // ---------------------------------------------------------------------------
-template <typename T>
-__device__ __forceinline__ Q<double> widen(const Q<T>& q) { return Q<double>{q.c0, q.c1, q.c2, q.c3}; }
 
 // The plaquette is evaluated in FP64 for both storages: an FP32 field's
 // plaquette is then the one the reference computes from the same links (as
@@ -630,6 +747,43 @@ __device__ __forceinline__ double plaq_term(const Planes<T>& P, const Geo& g, co
     un2 = qdag(widen(link(P, g, kk, s)));
     un0 = qmul(un1, un2);
     return 2.0 * un0.c0;
+}
+
+// The plaquette of a field in the reference's FP64 layout ([VOL][4][4], site
+// major) as given, before it is rounded to the storage precision: the value
+// readConfig compares with the file's (config_IO.c:609-614).  One thread per
+// lexicographic site; partial[block].
+__device__ __forceinline__ Q<double> aos_link(const double* __restrict__ aos, const Site& s, const Geo& g, int mu)
+{
+    const double* q = aos + 16 * (size_t)((((uint32_t)s.c[0] * g.L[1] + s.c[1]) * g.L[2] + s.c[2]) * (uint32_t)g.L[3] +
+                                          (uint32_t)s.c[3]) + 4 * mu;
+    return Q<double>{q[0], q[1], q[2], q[3]};
+}
+
+static __global__ void __launch_bounds__(BLOCK) k_plaquette_aos(const double* __restrict__ aos, Geo g,
+                                                                 DD* __restrict__ partial)
+{
+#pragma clang fp contract(off)
+    const uint32_t n = blockIdx.x * BLOCK + threadIdx.x;
+    DD acc = DD{0.0, 0.0};
+    if (n < 2 * g.vh) {
+        const Site s = lex_site(g, n);
+#pragma unroll
+        for (int jj = 0; jj < 3; jj++)
+#pragma unroll
+            for (int kk = jj + 1; kk < 4; kk++) {
+                Q<double> un0 = aos_link(aos, s, g, jj);
+                Q<double> un1 = aos_link(aos, shift(g, s, jj, 1), g, kk);
+                Q<double> un2 = qmul(un0, un1);
+                un0 = qdag(aos_link(aos, shift(g, s, kk, 1), g, jj));
+                un1 = qmul(un2, un0);
+                un2 = qdag(aos_link(aos, s, g, kk));
+                un0 = qmul(un1, un2);
+                acc = dd_add(acc, 2.0 * un0.c0);
+            }
+    }
+    acc = block_sum_dd(acc);
+    if (threadIdx.x == 0) partial[blockIdx.x] = acc;
 }
 
 // grid: x over (tile, parity) pairs; partial[p*tiles + tile]

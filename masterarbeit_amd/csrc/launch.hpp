@@ -53,6 +53,11 @@ template <typename T> struct Launch {
     // the straight-line fields of the loop gradient (lines_bytes of them, vjp_kernels.hpp)
     static hipError_t lines(hipStream_t s, const Geo& g, const RawPlanes& U, void* base, int R, int Tt);
     static hipError_t wloop_grad(hipStream_t s, const Geo& g, void* lines, int R, int Tt, const RawPlanes& G);
+    // one link at lexicographic site n: its staple (out: 4 doubles), and its
+    // heatbath from nu injected uniforms (link_out: 4 doubles, status: 2 ints)
+    static hipError_t staple_link(hipStream_t s, const Geo& g, const RawPlanes& P, uint32_t n, int mu, double* out);
+    static hipError_t local_heatbath(hipStream_t s, const Geo& g, const RawPlanes& P, uint32_t n, int mu, double beta,
+                                     const double* u, int nu, double* link_out, int* status);
 };
 
 // precision-independent launchers (lgt_f64.hip)
@@ -60,6 +65,8 @@ hipError_t launch_reduce_rows(hipStream_t s, const void* in, int nrows, int ncol
 hipError_t launch_hb_harness(hipStream_t s, int n, const double* staples, double beta, const double* uniforms,
                              const int* offsets, const int* lens, double* links, int* counts);
 hipError_t launch_or_harness(hipStream_t s, int n, const double* staples, const double* links_in, double* links);
+// plaquette of an FP64 field in the reference layout; partial: ceil(VOL/256) double-doubles
+hipError_t launch_plaquette_aos(hipStream_t s, const Geo& g, const double* aos, void* partial);
 hipError_t launch_metro_harness(hipStream_t s, int n, const double* staples, const double* links_in, double beta,
                                 double eps, const double* uniforms, double* links, int* accepted);
 

@@ -99,6 +99,22 @@ hipError_t Launch<T>::metropolis(hipStream_t s, const Geo& g, const RawPlanes& P
 }
 
 template <typename T>
+hipError_t Launch<T>::staple_link(hipStream_t s, const Geo& g, const RawPlanes& P, uint32_t n, int mu, double* out)
+{
+    hipLaunchKernelGGL(k_staple_link<T>, dim3(1), dim3(64), 0, s, planes<T>(P), g, n, mu, out);
+    return hipGetLastError();
+}
+
+template <typename T>
+hipError_t Launch<T>::local_heatbath(hipStream_t s, const Geo& g, const RawPlanes& P, uint32_t n, int mu, double beta,
+                                     const double* u, int nu, double* link_out, int* status)
+{
+    hipLaunchKernelGGL(k_local_heatbath<T>, dim3(1), dim3(64), 0, s, planes<T>(P), g, n, mu, beta, u, nu, link_out,
+                       status);
+    return hipGetLastError();
+}
+
+template <typename T>
 hipError_t Launch<T>::plaquette(hipStream_t s, const Geo& g, const RawPlanes& P, void* partial)
 {
     hipLaunchKernelGGL(k_plaquette<T>, dim3(2 * grid_x(g)), dim3(BLOCK), 0, s, planes<T>(P), g, (DD*)partial);

@@ -104,6 +104,10 @@ struct lgt_engine {
     hipEvent_t pin_ev[2] = {nullptr, nullptr};   // last DMA that used each chunk
     void* staging = nullptr;
     size_t staging_bytes = 0;
+    void* link_buf = nullptr;   // per-link calls: staple / new link (8 doubles), status (2 ints)
+    size_t link_buf_bytes = 0;
+    void* uni_buf = nullptr;    // per-link heatbath: the injected uniforms
+    size_t uni_buf_bytes = 0;
     int profile = 0;
     int order = -1;   // tile order of every kernel; -1: per kernel (see kgeo)
     int group = 0;    // LGT_OPT_GROUP (0: per kernel, see kgeo)
@@ -578,6 +582,8 @@ int lgt_destroy(lgt_engine* e)
     if (e->partial) hipFree(e->partial);
     if (e->dres) hipFree(e->dres);
     if (e->staging) hipFree(e->staging);
+    if (e->link_buf) hipFree(e->link_buf);
+    if (e->uni_buf) hipFree(e->uni_buf);
     if (e->hres) hipHostFree(e->hres);
     for (int b = 0; b < 2; b++) {
         if (e->pin[b]) hipHostFree(e->pin[b]);
@@ -639,6 +645,26 @@ int lgt_upload(lgt_engine* e, const double* aos)
     size_t bytes = sizeof(double) * 16 * (size_t)e->vol;
     TRY(ensure(e, &e->staging, &e->staging_bytes, bytes, "staging"));
     TRY(h2d(e, e->staging, aos, bytes));
+    TRY(run(e, LGT_K_FIELD, "from_aos",
+            [&] { return DISPATCH(e, from_aos(e->stream, e->geo, e->cur, (const double*)e->staging)); }));
+    HIP_TRY(hipStreamSynchronize(e->stream), "hipStreamSynchronize");
+    return LGT_OK;
+}
+
+int lgt_upload_plaquette(lgt_engine* e, const double* aos, double* plaq)
+{
+    if (!e || !aos || !plaq) return fail(LGT_EINVAL, "lgt_upload_plaquette: null argument");
+    TRY(set_dev(e));
+    size_t bytes = sizeof(double) * 16 * (size_t)e->vol;
+    TRY(ensure(e, &e->staging, &e->staging_bytes, bytes, "staging"));
+    TRY(h2d(e, e->staging, aos, bytes));
+    const int nb = (int)((2 * (size_t)e->geo.vh + 255) / 256);   // one partial per 256 sites
+    TRY(ensure(e, &e->partial, &e->partial_bytes, sizeof(double) * 2 * (size_t)nb, "partials"));
+    TRY(run(e, LGT_K_PLAQUETTE, "plaquette of the uploaded field",
+            [&] { return launch_plaquette_aos(e->stream, e->geo, (const double*)e->staging, e->partial); }));
+    double sum = 0.0;
+    TRY(reduce_to_host(e, 1, nb, &sum));
+    *plaq = sum / (double)(2 * 6 * e->vol);
     TRY(run(e, LGT_K_FIELD, "from_aos",
             [&] { return DISPATCH(e, from_aos(e->stream, e->geo, e->cur, (const double*)e->staging)); }));
     HIP_TRY(hipStreamSynchronize(e->stream), "hipStreamSynchronize");
@@ -715,15 +741,15 @@ int read_config_header(FILE* f, const char* path, const int expect[4], double* p
     return LGT_OK;
 }
 
-// uploads, recomputes the plaquette and applies readConfig's test (config_IO.c:609-614)
+// uploads, recomputes the plaquette of the links as read (FP64, before an FP32
+// engine rounds them) and applies readConfig's test (config_IO.c:609-614)
 int load_and_check(lgt_engine* e, const double* aos, double plaq0, double* plaq_file, double* plaq_calc, const char* who)
 {
-    TRY(lgt_upload(e, aos));
     double plaq = 0.0;
-    TRY(lgt_plaquette(e, &plaq));
+    TRY(lgt_upload_plaquette(e, aos, &plaq));
     if (plaq_file) *plaq_file = plaq0;
     if (plaq_calc) *plaq_calc = plaq;
-    const double eps = std::sqrt(6.0 * (double)e->vol) * (e->prec == LGT_FP64 ? DBL_EPSILON : (double)FLT_EPSILON);
+    const double eps = std::sqrt(6.0 * (double)e->vol) * DBL_EPSILON;
     if (!(std::fabs(plaq - plaq0) <= eps))
         return fail(LGT_EINVAL, "%s: plaquette test failed (file %.16g, field %.16g)", who, plaq0, plaq);
     return LGT_OK;
@@ -920,6 +946,55 @@ int lgt_plaquette(lgt_engine* e, double* plaq)
     double s = 0.0;
     TRY(reduce_to_host(e, 1, (int)(2 * nb), &s));
     *plaq = s / (double)(2 * 6 * e->vol);
+    return LGT_OK;
+}
+
+int lgt_staple(lgt_engine* e, int64_t site, int mu, double A[4])
+{
+    if (!e || !A) return fail(LGT_EINVAL, "lgt_staple: null argument");
+    if (site < 0 || site >= e->vol || mu < 0 || mu > 3)
+        return fail(LGT_EINVAL, "lgt_staple: link (%lld, %d) outside the lattice", (long long)site, mu);
+    TRY(set_dev(e));
+    TRY(ensure(e, &e->link_buf, &e->link_buf_bytes, 8 * sizeof(double) + 2 * sizeof(int), "link buffer"));
+    TRY(ensure_host(e, 8 * sizeof(double) + 2 * sizeof(int)));
+    TRY(run(e, LGT_K_FIELD, "staple", [&] {
+        return DISPATCH(e, staple_link(e->stream, e->geo, e->cur, (uint32_t)site, mu, (double*)e->link_buf));
+    }));
+    HIP_TRY(hipMemcpyAsync(e->hres, e->link_buf, 4 * sizeof(double), hipMemcpyDeviceToHost, e->stream), "hipMemcpyAsync");
+    HIP_TRY(hipStreamSynchronize(e->stream), "hipStreamSynchronize");
+    for (int k = 0; k < 4; k++) A[k] = e->hres[k];
+    return LGT_OK;
+}
+
+int lgt_local_heatbath(lgt_engine* e, int64_t site, int mu, double beta, const double* uniforms, int n_uniforms,
+                       double link[4], int* trials, int* needed)
+{
+    if (!e || !trials || !needed || (n_uniforms > 0 && !uniforms) || n_uniforms < 0)
+        return fail(LGT_EINVAL, "lgt_local_heatbath: bad arguments");
+    if (site < 0 || site >= e->vol || mu < 0 || mu > 3)
+        return fail(LGT_EINVAL, "lgt_local_heatbath: link (%lld, %d) outside the lattice", (long long)site, mu);
+    if (!(beta >= 0.0) || !std::isfinite(beta)) return fail(LGT_EINVAL, "lgt_local_heatbath: beta = %g", beta);
+    TRY(set_dev(e));
+    TRY(ensure(e, &e->link_buf, &e->link_buf_bytes, 8 * sizeof(double) + 2 * sizeof(int), "link buffer"));
+    TRY(ensure(e, &e->uni_buf, &e->uni_buf_bytes, sizeof(double) * (size_t)std::max(n_uniforms, 1), "uniforms"));
+    TRY(ensure_host(e, 8 * sizeof(double) + 2 * sizeof(int)));
+    if (n_uniforms > 0)
+        HIP_TRY(hipMemcpyAsync(e->uni_buf, uniforms, sizeof(double) * n_uniforms, hipMemcpyHostToDevice, e->stream),
+                "hipMemcpyAsync H2D");
+    double* dlink = (double*)e->link_buf + 4;
+    int* dstat = (int*)((double*)e->link_buf + 8);
+    TRY(run(e, LGT_K_HEATBATH, "local heatbath", [&] {
+        return DISPATCH(e, local_heatbath(e->stream, e->geo, e->cur, (uint32_t)site, mu, beta,
+                                          (const double*)e->uni_buf, n_uniforms, dlink, dstat));
+    }));
+    HIP_TRY(hipMemcpyAsync(e->hres, dlink, 4 * sizeof(double) + 2 * sizeof(int), hipMemcpyDeviceToHost, e->stream),
+            "hipMemcpyAsync");
+    HIP_TRY(hipStreamSynchronize(e->stream), "hipStreamSynchronize");
+    const int* st = (const int*)(e->hres + 4);
+    *trials = st[0];
+    *needed = st[1];
+    if (link && st[0] >= 0)
+        for (int k = 0; k < 4; k++) link[k] = e->hres[k];
     return LGT_OK;
 }
 

@@ -31,6 +31,13 @@ static int g_L[4];
 static int64_t g_vol = 0;
 static double *g_buf = NULL;
 static int g_nor = 0;
+static int g_prec = LGT_FP64;
+static double *g_uni = NULL;     /* uniforms of localHeatbathUpdate */
+static int g_uni_cap = 0;
+/* FP64 plaquette of pu's links at their last upload, while the device field
+ * still holds exactly those links */
+static int g_plaq_valid = 0;
+static double g_plaq = 0.0;
 
 /* coherence state */
 static int g_mode = -1;          /* -1: not chosen yet, 0: eager, 1: lazy */
@@ -71,6 +78,10 @@ void lgt_compat_release(void)
     g_eng = NULL;
     free(g_buf);
     g_buf = NULL;
+    free(g_uni);
+    g_uni = NULL;
+    g_uni_cap = 0;
+    g_plaq_valid = 0;
     g_host_dirty = 1;
     g_dev_dirty = 0;
 }
@@ -90,6 +101,7 @@ static void engine(void)
     int prec = LGT_FP64;
     const char *p = getenv("LGT_PRECISION");
     if (p && (!strcmp(p, "fp32") || !strcmp(p, "FP32"))) prec = LGT_FP32;
+    g_prec = prec;
     const char *n = getenv("LGT_N_OR");
     g_nor = n ? atoi(n) : 0;
     if (g_nor < 0) g_nor = 0;
@@ -121,7 +133,9 @@ static double *in_place(su2mat *u[][4])
     return base;
 }
 
-static void upload(su2mat *u[][4])
+/* uploads u; plaq (nullable) receives the plaquette of u's links as they
+ * are, in FP64 (lgt_upload_plaquette) */
+static void upload_plaq(su2mat *u[][4], double *plaq)
 {
     engine();
     double *f = in_place(u);
@@ -130,9 +144,18 @@ static void upload(su2mat *u[][4])
             for (int d = 0; d < 4; d++) memcpy(g_buf + 16 * s + 4 * d, u[s][d], 4 * sizeof(double));
         f = g_buf;
     }
-    check(lgt_upload(g_eng, f), "lgt_upload");
+    if (plaq) {
+        check(lgt_upload_plaquette(g_eng, f, plaq), "lgt_upload_plaquette");
+        g_plaq = *plaq;
+        g_plaq_valid = (u == pu);
+    } else {
+        check(lgt_upload(g_eng, f), "lgt_upload");
+        g_plaq_valid = 0;
+    }
     g_uploads++;
 }
+
+static void upload(su2mat *u[][4]) { upload_plaq(u, NULL); }
 
 static void download(su2mat *u[][4])
 {
@@ -166,6 +189,7 @@ static void pu_on_host(void)
 /* the device field (pu's links) was changed on the device */
 static void device_changed(void)
 {
+    g_plaq_valid = 0;
     if (lazy()) {
         g_dev_dirty = 1;
     } else {
@@ -185,6 +209,7 @@ static void run_on_other_field(su2mat *u[][4], int (*op)(int, double), int a, do
 
 void lgt_compat_host_modified(void)
 {
+    g_plaq_valid = 0;
     g_host_dirty = 1;
     g_dev_dirty = 0;
 }
@@ -204,6 +229,15 @@ void gaugefieldUpdate(int iup, int nup, int utype, int stype)
     char m2[] = "Unknown sweep type! Only sequential (SWEEP_TYPE=0) and random (SWEEP_TYPE=1) sweeps implemented!";
     error(utype != 0 && utype != 1, NAME, m1);
     error(stype != 0 && stype != 1, NAME, m2);
+    static int warned = 0;
+    if (stype != 0 && !warned) {
+        /* update.c:122-153 visits random sites in that mode; every GPU sweep
+         * visits the links in checkerboard order (same equilibrium) */
+        char w[] = "\nWarning: gaugefieldUpdate: sweep type %d (random site order) runs as the GPU's "
+                   "checkerboard sweep\n";
+        logging(w, stype);
+        warned = 1;
+    }
     double t1 = getTime();
     pu_on_device();
     double acc = 0.0;
@@ -217,10 +251,22 @@ void gaugefieldUpdate(int iup, int nup, int utype, int stype)
     logging(fmt, iup, utype, stype, acc, t2 - t1);
 }
 
+/* While the device field holds pu's links as the program last handed them
+ * over, the plaquette is that of pu's FP64 links (what the reference computes,
+ * and what readConfig tests against the file's, config_IO.c:609-614), also in
+ * FP32 storage, where the plaquette of the rounded links would fail that
+ * test; once the device has changed the field it is the device field's (FP64
+ * arithmetic in both storages). */
 double plaquette(void)
 {
     double p = 0.0;
-    pu_on_device();
+    engine();
+    if (!lazy() || g_host_dirty) {
+        upload_plaq(pu, &p);
+        g_host_dirty = 0;
+        return p;
+    }
+    if (g_plaq_valid) return g_plaq;
     check(lgt_plaquette(g_eng, &p), "lgt_plaquette");
     return p;
 }
@@ -288,6 +334,54 @@ void meas_topologicalcharge(double *q)
 {
     pu_on_device();
     check(lgt_qtop(g_eng, LGT_Q_CLOVER, q, NULL, NULL), "lgt_qtop");
+}
+
+/* staples(n, dir, &A) (metro.c:34-63) of pu's link (n, dir), on the device */
+void staples(uint64_t n, int dir, su2mat *stap)
+{
+    char m[] = "staples: link (%llu, %d) outside the lattice";
+    engine();
+    error(n >= (uint64_t)g_vol || dir < 0 || dir > 3, NAME, m, (unsigned long long)n, dir);
+    pu_on_device();
+    double A[4];
+    check(lgt_staple(g_eng, (int64_t)n, dir, A), "lgt_staple");
+    stap->c0 = A[0]; stap->c1 = A[1]; stap->c2 = A[2]; stap->c3 = A[3];
+}
+
+/* localHeatbathUpdate(n, dir, m) (heatbath.c:124-149): the new link is drawn
+ * on the device from uniforms of the program's ranlxd stream, drawn exactly
+ * as the reference draws them (3 on the degenerate branch; else 4 per trial,
+ * then 2), and written to the device field and to pu[n][dir].  Returns
+ * 1/trials, 0 on the degenerate branch. */
+double localHeatbathUpdate(uint64_t n, int dir, int m)
+{
+    (void)m;
+    char m1[] = "localHeatbathUpdate: link (%llu, %d) outside the lattice";
+    char m2[] = "localHeatbathUpdate: cannot allocate %d uniforms";
+    engine();
+    error(n >= (uint64_t)g_vol || dir < 0 || dir > 3, NAME, m1, (unsigned long long)n, dir);
+    pu_on_device();
+    int have = 0, trials = -1, needed = 0;
+    double link[4];
+    for (;;) {
+        check(lgt_local_heatbath(g_eng, (int64_t)n, dir, runParams.beta, g_uni, have, link, &trials, &needed),
+              "lgt_local_heatbath");
+        if (trials >= 0) break;
+        if (needed > g_uni_cap) {
+            int cap = needed > 2 * g_uni_cap ? needed + 64 : 2 * g_uni_cap;
+            double *u = (double *)realloc(g_uni, sizeof(double) * (size_t)cap);
+            error(u == NULL, NAME, m2, cap);
+            g_uni = u;
+            g_uni_cap = cap;
+        }
+        ranlxd(g_uni + have, needed - have);
+        have = needed;
+    }
+    g_plaq_valid = 0;
+    /* pu[n][dir] now holds the device's link: eager mode stays current, and
+     * lazy mode keeps its marks (the device field is current) */
+    pu[n][dir]->c0 = link[0]; pu[n][dir]->c1 = link[1]; pu[n][dir]->c2 = link[2]; pu[n][dir]->c3 = link[3];
+    return trials > 0 ? 1.0 / trials : 0.0;
 }
 
 static int project_op(int unused_a, double unused_b)

@@ -22,6 +22,12 @@ hipError_t launch_hb_harness(hipStream_t s, int n, const double* staples, double
     return hipGetLastError();
 }
 
+hipError_t launch_plaquette_aos(hipStream_t s, const Geo& g, const double* aos, void* partial)
+{
+    hipLaunchKernelGGL(k_plaquette_aos, dim3((2 * g.vh + BLOCK - 1) / BLOCK), dim3(BLOCK), 0, s, aos, g, (DD*)partial);
+    return hipGetLastError();
+}
+
 hipError_t launch_or_harness(hipStream_t s, int n, const double* staples, const double* links_in, double* links)
 {
     hipLaunchKernelGGL(k_or_harness, dim3((n + BLOCK - 1) / BLOCK), dim3(BLOCK), 0, s, n, staples, links_in, links);

@@ -40,6 +40,9 @@ SIGNATURES = {
     "lgt_init_field": (C.c_int, [C.c_void_p, C.c_int]),
     "lgt_upload": (C.c_int, [C.c_void_p, _dp]),
     "lgt_download": (C.c_int, [C.c_void_p, _dp]),
+    "lgt_upload_plaquette": (C.c_int, [C.c_void_p, _dp, _dp]),
+    "lgt_staple": (C.c_int, [C.c_void_p, C.c_int64, C.c_int, _dp]),
+    "lgt_local_heatbath": (C.c_int, [C.c_void_p, C.c_int64, C.c_int, C.c_double, _dp, C.c_int, _dp, _ip, _ip]),
     "lgt_write_config": (C.c_int, [C.c_void_p, C.c_char_p]),
     "lgt_read_config": (C.c_int, [C.c_void_p, C.c_char_p, _dp, _dp]),
     "lgt_read_config_doubled": (C.c_int, [C.c_void_p, C.c_char_p, _dp, _dp]),
@@ -166,6 +169,34 @@ class Engine:
         if U.size != 16 * self.vol:
             raise ValueError("field size mismatch")
         _check(self._lib.lgt_upload(self._h, _ptr(U)), "lgt_upload")
+
+    def upload_plaquette(self, U):
+        """Uploads U and returns the FP64 plaquette of its links as given."""
+        U = np.ascontiguousarray(U, dtype=np.float64)
+        if U.size != 16 * self.vol:
+            raise ValueError("field size mismatch")
+        p = C.c_double(0)
+        _check(self._lib.lgt_upload_plaquette(self._h, _ptr(U), C.byref(p)), "lgt_upload_plaquette")
+        return p.value
+
+    def staple(self, site, mu):
+        """staples(n, mu) of the current field at lexicographic site n."""
+        A = np.zeros(4)
+        _check(self._lib.lgt_staple(self._h, int(site), int(mu), _ptr(A)), "lgt_staple")
+        return A
+
+    def local_heatbath(self, site, mu, beta, uniforms):
+        """localHeatbathUpdate of one link with the given uniforms: returns
+        (trials, needed, new link); trials = -1 (and no change) when the
+        uniforms run out, needed = how many the next call needs."""
+        u = np.ascontiguousarray(uniforms, dtype=np.float64).ravel()
+        link = np.zeros(4)
+        trials = C.c_int(0)
+        needed = C.c_int(0)
+        _check(self._lib.lgt_local_heatbath(self._h, int(site), int(mu), beta, _ptr(u) if u.size else None,
+                                            int(u.size), _ptr(link), C.byref(trials), C.byref(needed)),
+               "lgt_local_heatbath")
+        return trials.value, needed.value, (link if trials.value >= 0 else None)
 
     def write_config(self, path):
         _check(self._lib.lgt_write_config(self._h, os.fsencode(path)), "lgt_write_config")

@@ -439,12 +439,19 @@ def main():
     ap.add_argument("study", choices=["c1", "c2", "c3", "c4", "c5", "c6", "c7", "c8", "all"])
     ap.add_argument("--quick", action="store_true", help="fewer sweeps (a functional check)")
     ap.add_argument("--seeds", type=int, default=8, help="C6: independent chains per precision")
+    ap.add_argument("--nmeas", type=int, default=5000, help="C8: measured configurations per chain")
+    ap.add_argument("--every", type=int, default=2, help="C8: sweeps between measurements")
     a = ap.parse_args()
     E.load_library()
     todo = ["c1", "c2", "c3", "c4", "c5", "c6", "c7", "c8"] if a.study == "all" else [a.study]
     for s in todo:
         log(f"[{s}] start")
-        r = c6(a.quick, a.seeds) if s == "c6" else globals()[s](a.quick)
+        if s == "c6":
+            r = c6(a.quick, a.seeds)
+        elif s == "c8":
+            r = c8(a.quick, a.nmeas, a.every)
+        else:
+            r = globals()[s](a.quick)
         print(json.dumps(r), flush=True)
         log(f"[{s}] done in {r['seconds']:.1f} s")
 

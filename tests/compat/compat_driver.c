@@ -4,7 +4,13 @@
  * logging/error/timer/RNG services and calls the reference's entry points,
  * which liblgt_compat.so serves from the GPU engine.  Test infrastructure.
  *
- * usage: compat_driver L0 L1 L2 L3 field.bin outdir
+ * usage: compat_driver L0 L1 L2 L3 field.bin outdir [perlink]
+ *
+ * With "perlink" it runs the reference's sweep loop itself (update.c:122-166:
+ * lexicographic sites, directions 0..3, localHeatbathUpdate per link, then
+ * project_gfield_to_sun) through the per-link entry points, after the
+ * staples of a few links, with ranlxd drawing from a 64-bit LCG that the test
+ * reproduces.
  */
 #include <stdarg.h>
 #include <stdio.h>
@@ -52,11 +58,25 @@ double getTime(void)
     return t.tv_sec + 1e-6 * t.tv_usec;
 }
 
-/* deterministic stand-in for the program's RANLUX stream: 0.25, 0.5, 0.75, ... */
+/* deterministic stand-ins for the program's RANLUX stream: 0.25, 0.5, 0.75,
+ * ...; or, once g_lcg is set, x <- 6364136223846793005 x + 1442695040888963407
+ * (from x = 12345), r = (x >> 11) / 2^53 */
+static int g_lcg = 0;
+static unsigned long long g_lcg_x = 12345ULL;
+static long long g_draws = 0;
+
 void ranlxd(double r[], int n)
 {
     static int k = 0;
-    for (int i = 0; i < n; i++) r[i] = 0.25 * (double)((k++ % 3) + 1);
+    for (int i = 0; i < n; i++) {
+        if (g_lcg) {
+            g_lcg_x = 6364136223846793005ULL * g_lcg_x + 1442695040888963407ULL;
+            r[i] = (double)(g_lcg_x >> 11) * (1.0 / 9007199254740992.0);
+        } else {
+            r[i] = 0.25 * (double)((k++ % 3) + 1);
+        }
+        g_draws++;
+    }
 }
 
 /* field lifecycle (compat_lifecycle.c, a separate object as in the reference) */
@@ -73,10 +93,42 @@ static void write_field(const char *dir, const char *name)
     writeConfig(path);
 }
 
+/* the reference's sweep, link by link through the exported entry points */
+static int perlink(const char *field, const char *out)
+{
+    char path[1024];
+    readConfig((char *)field);
+    runParams.beta = 3.09;
+    (void)plaquette();              /* creates the engine: its seed draws come first */
+    snprintf(path, sizeof(path), "%s/staples.bin", out);
+    FILE *f = fopen(path, "wb");
+    for (long n = 0; n < driver_vol; n += 37)
+        for (int d = 0; d < 4; d++) {
+            su2mat A;
+            staples((uint64_t)n, d, &A);
+            fwrite(&A, sizeof(double), 4, f);
+        }
+    fclose(f);
+    g_lcg = 1;
+    const long long d0 = g_draws;
+    double nsum = 0.0;
+    for (long n = 0; n < driver_vol; n++)
+        for (int d = 0; d < 4; d++) nsum += localHeatbathUpdate((uint64_t)n, d, 1);
+    project_gfield_to_sun(pu);
+    write_field(out, "perlink_sweep.bin");
+    snprintf(path, sizeof(path), "%s/summary.txt", out);
+    f = fopen(path, "w");
+    fprintf(f, "acc %.17g\nuniforms %lld\n", nsum / (4.0 * (double)driver_vol), g_draws - d0);
+    fclose(f);
+    releaseGaugeField();
+    lgt_compat_release();
+    return 0;
+}
+
 int main(int argc, char **argv)
 {
-    if (argc != 7) {
-        fprintf(stderr, "usage: %s L0 L1 L2 L3 field.bin outdir\n", argv[0]);
+    if (argc != 7 && !(argc == 8 && !strcmp(argv[7], "perlink"))) {
+        fprintf(stderr, "usage: %s L0 L1 L2 L3 field.bin outdir [perlink]\n", argv[0]);
         return 2;
     }
     long vol = 1;
@@ -89,6 +141,7 @@ int main(int argc, char **argv)
     snprintf(LOG_FILE, sizeof(LOG_FILE), "%s/log.txt", out);
     driver_vol = vol;
     initGaugeField(0);
+    if (argc == 8) return perlink(argv[5], out);
     readConfig(argv[5]);
     runParams.beta = 3.09;
 

@@ -76,12 +76,35 @@ def test_engine_reads_reference_file(lgt, tmp_path, prec):
         p0, p1 = e.read_config(tmp_path / "conf")
         V = e.download()
     assert p0 == ref.lib.ref_plaquette()
+    # the reference's own test in both storages: the plaquette of the links as
+    # read, before FP32 storage rounds them
+    assert abs(p1 - p0) <= np.sqrt(6 * np.prod(L)) * np.finfo(np.float64).eps
     if prec == 0:
         np.testing.assert_array_equal(V, U)
-        assert abs(p1 - p0) <= np.sqrt(6 * np.prod(L)) * np.finfo(np.float64).eps
     else:
         np.testing.assert_allclose(V, U, atol=1e-7)
-        assert abs(p1 - p0) <= np.sqrt(6 * np.prod(L)) * np.finfo(np.float32).eps
+
+
+@pytest.mark.parametrize("prec", [0, 1])
+def test_upload_plaquette_is_that_of_the_fp64_links(lgt, prec):
+    """lgt_upload_plaquette returns the plaquette of the FP64 links as given
+    (the oracle's, to the rounding of the sum); for FP32 storage the plaquette
+    of the rounded links on the device differs from it by far more than
+    readConfig's tolerance sqrt(6 VOL) DBL_EPSILON."""
+    o = Oracle(L)
+    U = o.hot(41)
+    o.engine_sweep(U, 3.09, 1, 41, 0)
+    ref = o.plaquette(U, acc=True)
+    with engine(lgt, prec) as e:
+        p = e.upload_plaquette(U)
+        p_dev = e.plaquette()
+    assert p == pytest.approx(ref, abs=1e-15)
+    tol = np.sqrt(6 * np.prod(L)) * np.finfo(np.float64).eps
+    if prec == 0:
+        assert abs(p_dev - ref) <= 1e-15
+    else:
+        print(f"FP32 storage: plaquette of the rounded links - FP64 plaquette = {p_dev - ref:.2e} (tolerance {tol:.1e})")
+        assert abs(p_dev - ref) > tol
 
 
 def test_read_config_rejects_mismatch(lgt, tmp_path):
