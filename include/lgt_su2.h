@@ -190,6 +190,14 @@ int lgt_staple(lgt_engine *e, int64_t site, int mu, double A[4]);
  * reproduces the reference's draws. */
 int lgt_local_heatbath(lgt_engine *e, int64_t site, int mu, double beta, const double *uniforms, int n_uniforms,
                        double link[4], int *trials, int *needed);
+/* No reference counterpart: the random gauge transformation
+ * U'_mu(x) = g(x) U_mu(x) g(x+mu)^+ of the field, in place, with g(x)
+ * Haar-uniform from the engine's counter-based RNG under `seed` (computed in
+ * FP64, links rounded to the storage precision).  The plaquette, Wilson loops
+ * and charges are invariant under it and APE smearing is covariant: a check
+ * that costs one pass over the field at any lattice size.  g_out (nullable)
+ * receives g(x) for every site in lexicographic order (VOL quaternions). */
+int lgt_gauge_transform(lgt_engine *e, uint64_t seed, double *g_out);
 /* sweep counter of the RNG (advanced by lgt_update; set to resume a chain) */
 int lgt_set_sweep_index(lgt_engine *e, uint32_t sweep);
 uint32_t lgt_get_sweep_index(const lgt_engine *e);

@@ -169,6 +169,19 @@ __global__ void __launch_bounds__(BLOCK) k_hot(Planes<T> P, Geo g, uint64_t seed
     qstore<T>(P.p[mu][p], i, q);
 }
 
+// site of lexicographic index n (n = z + L3(y + L2(x + L1 t)))
+__device__ __forceinline__ Site lex_site(const Geo& g, uint32_t n)
+{
+    Site s;
+    s.c[3] = (int)(n % (uint32_t)g.L[3]);
+    n /= (uint32_t)g.L[3];
+    s.c[2] = (int)(n % (uint32_t)g.L[2]);
+    n /= (uint32_t)g.L[2];
+    s.c[1] = (int)(n % (uint32_t)g.L[1]);
+    s.c[0] = (int)(n / (uint32_t)g.L[1]);
+    return s;
+}
+
 // lexicographic site of cb index i, parity p
 __device__ __forceinline__ uint64_t lex_index(const Geo& g, const Site& s)
 {
@@ -223,6 +236,47 @@ __global__ void __launch_bounds__(BLOCK) k_to_aos(Planes<T> P, Geo g, double* __
         a[2 * mu] = lo;
         a[2 * mu + 1] = hi;
     }
+}
+
+// ---------------------------------------------------------------------------
+// random gauge transformation U'_mu(x) = g(x) U_mu(x) g(x+mu)^+ (no reference
+// counterpart; observables are invariant under it and APE smearing is
+// covariant, which tests check at any lattice size).  g(x) is Haar-uniform
+// (Shoemake's method) from the Philox stream of site x under the given seed,
+// in FP64; every link depends on its own old value only, so the transform
+// runs in place.
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ Q<double> gauge_element(const Geo& g, const Site& s, uint64_t seed)
+{
+    RngSrc64 src(seed, cb_index(g, s), (uint32_t)(parity(s) << 2 | (KIND_GAUGE << 3)), 0u);
+    double r1 = src.next(), r2 = src.next(), r3 = src.next();
+    double s1 = sqrt(1.0 - r1), s2 = sqrt(r1);
+    double t1 = 2.0 * PI_D * r2, t2 = 2.0 * PI_D * r3;
+    return Q<double>{s2 * cos(t2), s1 * sin(t1), s1 * cos(t1), s2 * sin(t2)};
+}
+
+// grid: x over cb sites, y = mu*2 + parity
+template <typename T>
+__global__ void __launch_bounds__(BLOCK) k_gauge(Planes<T> P, Geo g, uint64_t seed)
+{
+    const uint32_t i = blockIdx.x * BLOCK + threadIdx.x;
+    const int mu = blockIdx.y >> 1, p = blockIdx.y & 1;
+    if (i >= g.vh) return;
+    const Site s = cb_site(g, i, p);
+    const Q<double> gx = gauge_element(g, s, seed), gy = gauge_element(g, shift(g, s, mu, 1), seed);
+    const Q<T> u = qload<T>(P.p[mu][p], i);
+    const Q<double> v = qmul_dag(qmul(gx, Q<double>{u.c0, u.c1, u.c2, u.c3}), gy);
+    qstore<T>(P.p[mu][p], i, Q<T>{(T)v.c0, (T)v.c1, (T)v.c2, (T)v.c3});
+}
+
+// g(x) of every lexicographic site (VOL quaternions, for checks)
+template <typename T>
+__global__ void __launch_bounds__(BLOCK) k_gauge_elements(Geo g, uint64_t seed, double* __restrict__ out)
+{
+    const uint32_t n = blockIdx.x * BLOCK + threadIdx.x;
+    if (n >= 2 * g.vh) return;
+    const Q<double> q = gauge_element(g, lex_site(g, n), seed);
+    out[4 * (size_t)n] = q.c0; out[4 * (size_t)n + 1] = q.c1; out[4 * (size_t)n + 2] = q.c2; out[4 * (size_t)n + 3] = q.c3;
 }
 
 template <typename T>
@@ -635,18 +689,6 @@ __device__ __forceinline__ Q<double> widen(const Q<T>& q) { return Q<double>{q.c
 // lexicographic site n, for programs that call them link by link.  One
 // thread; FP32 fields compute the staple in FP32 and run the FP64 sampler.
 // ---------------------------------------------------------------------------
-__device__ __forceinline__ Site lex_site(const Geo& g, uint32_t n)
-{
-    Site s;
-    s.c[3] = (int)(n % (uint32_t)g.L[3]);
-    n /= (uint32_t)g.L[3];
-    s.c[2] = (int)(n % (uint32_t)g.L[2]);
-    n /= (uint32_t)g.L[2];
-    s.c[1] = (int)(n % (uint32_t)g.L[1]);
-    s.c[0] = (int)(n / (uint32_t)g.L[1]);
-    return s;
-}
-
 template <typename T>
 __device__ __forceinline__ Q<T> update_staple_mu(const Planes<T>& P, const Geo& g, const Site& s, int p, int mu)
 {

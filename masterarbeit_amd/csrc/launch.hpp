@@ -26,6 +26,9 @@ template <typename T> struct Launch {
     static hipError_t from_aos(hipStream_t s, const Geo& g, const RawPlanes& P, const double* aos);
     static hipError_t to_aos(hipStream_t s, const Geo& g, const RawPlanes& P, double* aos);
     static hipError_t project(hipStream_t s, const Geo& g, const RawPlanes& P);
+    // random gauge transformation in place; gauge_elements: g(x) per lexicographic site
+    static hipError_t gauge(hipStream_t s, const Geo& g, const RawPlanes& P, uint64_t seed);
+    static hipError_t gauge_elements(hipStream_t s, const Geo& g, uint64_t seed, double* out);
     // acc_partial: nullable, receives ceil(vh/256) block sums of 1/count
     static hipError_t heatbath(hipStream_t s, const Geo& g, const RawPlanes& P, int mu, int parity, double beta,
                                uint64_t seed, uint32_t sweep, void* acc_partial);

@@ -52,6 +52,20 @@ hipError_t Launch<T>::project(hipStream_t s, const Geo& g, const RawPlanes& P)
 }
 
 template <typename T>
+hipError_t Launch<T>::gauge(hipStream_t s, const Geo& g, const RawPlanes& P, uint64_t seed)
+{
+    hipLaunchKernelGGL(k_gauge<T>, dim3(grid_x(g), 8), dim3(BLOCK), 0, s, planes<T>(P), g, seed);
+    return hipGetLastError();
+}
+
+template <typename T>
+hipError_t Launch<T>::gauge_elements(hipStream_t s, const Geo& g, uint64_t seed, double* out)
+{
+    hipLaunchKernelGGL(k_gauge_elements<T>, dim3((2 * g.vh + BLOCK - 1) / BLOCK), dim3(BLOCK), 0, s, g, seed, out);
+    return hipGetLastError();
+}
+
+template <typename T>
 hipError_t Launch<T>::heatbath(hipStream_t s, const Geo& g, const RawPlanes& P, int mu, int parity, double beta,
                                uint64_t seed, uint32_t sweep, void* acc_partial)
 {

@@ -889,6 +889,23 @@ int lgt_project(lgt_engine* e)
     return run(e, LGT_K_PROJECT, "project", [&] { return DISPATCH(e, project(e->stream, e->geo, e->cur)); });
 }
 
+int lgt_gauge_transform(lgt_engine* e, uint64_t seed, double* g_out)
+{
+    if (!e) return fail(LGT_EINVAL, "lgt_gauge_transform: null engine");
+    TRY(set_dev(e));
+    TRY(run(e, LGT_K_FIELD, "gauge transformation",
+            [&] { return DISPATCH(e, gauge(e->stream, e->geo, e->cur, seed)); }));
+    if (g_out) {
+        const size_t bytes = sizeof(double) * 4 * (size_t)e->vol;
+        TRY(ensure(e, &e->staging, &e->staging_bytes, sizeof(double) * 16 * (size_t)e->vol, "staging"));
+        TRY(run(e, LGT_K_FIELD, "gauge elements",
+                [&] { return DISPATCH(e, gauge_elements(e->stream, e->geo, seed, (double*)e->staging)); }));
+        TRY(d2h(e, g_out, e->staging, bytes));
+    }
+    HIP_TRY(hipStreamSynchronize(e->stream), "hipStreamSynchronize");
+    return LGT_OK;
+}
+
 int lgt_set_sweep_index(lgt_engine* e, uint32_t sweep)
 {
     if (!e) return fail(LGT_EINVAL, "lgt_set_sweep_index: null engine");

@@ -251,7 +251,7 @@ __device__ __forceinline__ double u53(uint32_t hi, uint32_t lo)
 __device__ __forceinline__ float u24(uint32_t x) { return (float)(x >> 8) * 0x1p-24f; }
 
 // counter layout of a link stream: (cb site, mu | parity<<2 | kind<<3, sweep, draw)
-enum { KIND_HB = 0, KIND_HOT = 1, KIND_METRO = 2 };
+enum { KIND_HB = 0, KIND_HOT = 1, KIND_METRO = 2, KIND_GAUGE = 3 };
 
 struct LinkRng {
     U32x4 ctr;

@@ -51,6 +51,7 @@ SIGNATURES = {
     "lgt_overrelax_subsweep": (C.c_int, [C.c_void_p, C.c_int, C.c_int]),
     "lgt_project": (C.c_int, [C.c_void_p]),
     "lgt_set_sweep_index": (C.c_int, [C.c_void_p, C.c_uint32]),
+    "lgt_gauge_transform": (C.c_int, [C.c_void_p, C.c_uint64, _dp]),
     "lgt_get_sweep_index": (C.c_uint32, [C.c_void_p]),
     "lgt_plaquette": (C.c_int, [C.c_void_p, _dp]),
     "lgt_gauge_action": (C.c_int, [C.c_void_p, C.c_double, _dp]),
@@ -245,6 +246,14 @@ class Engine:
 
     def overrelax_subsweep(self, mu, parity):
         _check(self._lib.lgt_overrelax_subsweep(self._h, mu, parity), "lgt_overrelax_subsweep")
+
+    def gauge_transform(self, seed, elements=False):
+        """Random gauge transformation of the field in place; returns g(x)
+        per lexicographic site if elements is set."""
+        G = np.empty((self.vol, 4)) if elements else None
+        _check(self._lib.lgt_gauge_transform(self._h, C.c_uint64(seed), _ptr(G) if elements else None),
+               "lgt_gauge_transform")
+        return G
 
     def project(self):
         _check(self._lib.lgt_project(self._h), "lgt_project")

@@ -328,19 +328,19 @@ def c7(quick=False):
 
 
 # The reference's own 24^4 chains at beta 3.09 (heatbath, 50 000 stored
-# configurations each from a cold and a hot start): mean over configurations
-# 1000..49999 and error of the mean from the means of 49 blocks of 1000
-# configurations, cold and hot combined; Thermalisation/
-# 24x24x24x24_b3.09_WL_<R>_<T>_{cold,hot}.dat, and W(10,16) after 100 spatial
-# APE steps on configurations 1000..9999 (..._WL_10_16_{cold,hot}_NAPE_100.dat;
-# its smearing parameter is not archived with it).  The smeared series of the
-# two starts differ by 1.4e-4, five times their block error (the hot chain's
-# first 10 000 configurations have 2 tau_int = 28), so its error here is half
-# that difference.  The same figures with autocorrelation-time errors per
-# start are in tests/golden/ref_therm24.json.
-W_REF24 = {(1, 1): (0.7325384, 1.3e-6), (3, 3): (0.1955563, 6.7e-6), (5, 5): (0.0442735, 7.9e-6),
-           (7, 7): (0.0095009, 5.9e-6), (9, 9): (0.0019756, 3.6e-6), (10, 16): (8.3275e-05, 1.9e-6)}
-W_REF24_APE100 = (0.0017639, 7.0e-5)
+# configurations each from a cold and from a hot start;
+# Thermalisation/24x24x24x24_b3.09_WL_<R>_<T>_{cold,hot}.dat): mean over
+# configurations 10000..49999 of the two starts and error from the means of
+# blocks of 1000 configurations (its hot chain still drifts upwards in the
+# larger loops over configurations 1000..9999).  W(10,16) after 100 spatial
+# APE steps: configurations 1000..9999 (..._WL_10_16_{cold,hot}_NAPE_100.dat,
+# the only ones smeared; the smearing parameter is not archived with them),
+# whose two starts differ by 1.4e-4, five times their block error, so the
+# error here is half that difference.  The same figures with
+# autocorrelation-time errors per start are in tests/golden/ref_therm24.json.
+W_REF24 = {(1, 1): (0.73253802, 1.4e-6), (3, 3): (0.19556128, 6.4e-6), (5, 5): (0.044281381, 7.3e-6),
+           (7, 7): (0.0095079745, 5.7e-6), (9, 9): (0.0019805584, 3.4e-6), (10, 16): (8.4210018e-05, 2.1e-6)}
+W_REF24_APE100 = (0.0017639191, 6.8e-5)
 
 
 def c8(quick=False, nmeas=5000, every=2, therm=2000, ape_every=10):

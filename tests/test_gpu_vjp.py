@@ -223,6 +223,19 @@ def test_vjp_400_steps_48_finite_differences(lgt):
             worst = max(worst, rel)
             assert rel < 1e-6 or abs(an - fd) < 1e-9, (n, mu, an, fd)
         print(f"48^4 APE-400 adjoint vs finite differences: worst relative deviation {worst:.2e}")
+    # the same gradient in FP32 storage (FP32 links and arithmetic, FP64
+    # checkpoint-boundary and objective sums) against the FP64 one
+    with lgt.Engine(L, 1, 11) as e:
+        e.upload(U)
+        e.ape_vjp_forward(400, 0.25)
+        e.wloop_grad(R, T, host=False)
+        G32 = e.ape_vjp_backward()
+    scale = float(np.max(np.abs(G)))
+    dmax = float(np.max(np.abs(G32 - G))) / scale
+    drms = float(np.sqrt(np.mean((G32 - G) ** 2) / np.mean(G ** 2)))
+    print(f"48^4 APE-400 adjoint, FP32 storage - FP64: max {dmax:.2e} of the largest component, rms {drms:.2e} "
+          f"relative")
+    assert dmax <= 1e-4 and drms <= 1e-5
 
 
 def test_vjp_rejects_bad_calls(lgt):

@@ -82,3 +82,17 @@ def test_study_quaternion_helpers_match_the_link_algebra():
         # exp(eps X) = cos(eps |x|) + sin(eps |x|) x/|x|, and d/deps at 0 is X
         d = (su2_exp(X, 1e-6) - su2_exp(X, -1e-6)) / 2e-6
         np.testing.assert_allclose(d, X, rtol=1e-8, atol=1e-8)
+
+
+def test_reference_24_constants_match_fixture():
+    """The reference's 24^4 equilibrium values quoted in the study driver
+    (W_REF24, W_REF24_APE100) are those derived from the fixture of the
+    reference's own series, to the digits quoted."""
+    from masterarbeit_amd.studies import W_REF24, W_REF24_APE100
+    from test_gpu_therm24 import reference_values
+    ref = reference_values()
+    for (R, T), (m, e) in W_REF24.items():
+        rm, re = ref[f"{R},{T}"]
+        assert m == pytest.approx(rm, rel=1e-7) and e == pytest.approx(re, rel=0.05), (R, T, rm, re)
+    rm, re = ref["ape100"]
+    assert W_REF24_APE100[0] == pytest.approx(rm, rel=1e-4) and W_REF24_APE100[1] == pytest.approx(re, rel=0.05)
