@@ -278,6 +278,26 @@ int lgt_overrelax_harness(int n, const double *staples, const double *links_in, 
 int lgt_metropolis_harness(int n, const double *staples, const double *links_in, double beta, double eps,
                            const double *uniforms, double *links_out, int *accepted);
 
+/* ---- allocation-free plaquette --------------------------------------------
+ * No reference counterpart.  For callers that cannot allocate device memory
+ * (code under HIP graph capture, tight serving loops, harnesses that forbid
+ * hipMalloc): the plaquette of lgt_upload_plaquette (plaquette.c:67-101 on an
+ * FP64 field in the reference layout) with the field already in device memory
+ * and the result written to device memory, enqueued on `stream` (a
+ * hipStream_t; NULL: the null stream) without any allocation, and no engine.
+ * field_dev: VOL*16 doubles (site major, then direction, c0..c3); plaq_dev:
+ * one double; both device pointers owned by the caller, and the only
+ * exception to "no device pointers" in this ABI.  extents as lgt_create, with
+ * VOL <= LGT_NOALLOC_MAX_VOL (LGT_EINVAL otherwise).  The partial sums live in
+ * a static device array of (LGT_NOALLOC_MAX_VOL/256 + 1) * 16 bytes = 5.3 MB
+ * that every device the library runs on keeps until the process ends, shared
+ * by all calls on that device: calls that overlap on one device (other
+ * streams, other host threads) race, so serialise them.  Returns after
+ * enqueueing; the result is complete when `stream` reaches it, and equals
+ * lgt_upload_plaquette's for the same links bit for bit. */
+#define LGT_NOALLOC_MAX_VOL (96LL * 96 * 96 * 96)
+int lgt_plaquette_aos_noalloc(const int extents[4], const double *field_dev, double *plaq_dev, void *stream);
+
 /* ---- timing --------------------------------------------------------------- */
 /* With LGT_OPT_PROFILE on, every launch is bracketed by HIP events on the
  * engine's stream; this returns the summed kernel time and launch count of

@@ -75,6 +75,9 @@ hipError_t launch_metro_harness(hipStream_t s, int n, const double* staples, con
 
 inline uint32_t grid_x(const Geo& g) { return (g.vh + 255u) / 256u; }
 
+// records the message lgt_last_error returns (no allocation); returns code
+int set_error(int code, const char* fmt, ...);
+
 extern template struct Launch<double>;
 extern template struct Launch<float>;
 

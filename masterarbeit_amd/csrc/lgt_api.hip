@@ -13,7 +13,6 @@
 #include <algorithm>
 #include <map>
 #include <thread>
-#include <string>
 #include <vector>
 
 #include "../../include/lgt_su2.h"
@@ -23,16 +22,16 @@ using namespace lgt;
 
 namespace {
 
-thread_local std::string g_last_error;
+// message of the last failed call on this host thread (a fixed buffer: the
+// allocation-free entry point reports through it too)
+thread_local char g_last_error[512];
 
 int fail(int code, const char* fmt, ...)
 {
-    char buf[512];
     va_list ap;
     va_start(ap, fmt);
-    vsnprintf(buf, sizeof(buf), fmt, ap);
+    vsnprintf(g_last_error, sizeof(g_last_error), fmt, ap);
     va_end(ap);
-    g_last_error = buf;
     return code;
 }
 
@@ -80,6 +79,17 @@ struct VjpState {
 };
 
 }  // namespace
+
+namespace lgt {
+int set_error(int code, const char* fmt, ...)
+{
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(g_last_error, sizeof(g_last_error), fmt, ap);
+    va_end(ap);
+    return code;
+}
+}  // namespace lgt
 
 struct lgt_engine {
     int dev = 0;
@@ -497,7 +507,7 @@ const char* lgt_strerror(int status)
     }
 }
 
-const char* lgt_last_error(void) { return g_last_error.c_str(); }
+const char* lgt_last_error(void) { return g_last_error; }
 
 int lgt_create(const int extents[4], int precision, uint64_t seed, lgt_engine** out)
 {

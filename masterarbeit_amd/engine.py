@@ -66,6 +66,7 @@ SIGNATURES = {
     "lgt_overrelax_harness": (C.c_int, [C.c_int, _dp, _dp, _dp]),
     "lgt_metropolis_update": (C.c_int, [C.c_void_p, C.c_int, C.c_double, C.c_double, _dp]),
     "lgt_metropolis_harness": (C.c_int, [C.c_int, _dp, _dp, C.c_double, C.c_double, _dp, _dp, _ip]),
+    "lgt_plaquette_aos_noalloc": (C.c_int, [_ip, C.c_void_p, C.c_void_p, C.c_void_p]),
     "lgt_profile_read": (C.c_int, [C.c_void_p, C.c_int, _dp, C.POINTER(C.c_int64)]),
     "lgt_profile_reset": (C.c_int, [C.c_void_p]),
     "lgt_sync": (C.c_int, [C.c_void_p]),
