@@ -1115,152 +1115,9 @@ __global__ void __launch_bounds__(WL<T>::THREADS) k_wloops(Planes<T> P, Planes<T
 }
 
 // ---------------------------------------------------------------------------
-// improved clover charge (LGT_Code/modules/obs/topologicalcharge.c:18-199)
+// topological charges (LGT_Code/modules/obs/topologicalcharge.c): the 1x1
+// clover here, the improved clover from line fields below
 // ---------------------------------------------------------------------------
-// (X - X^+)/2 (topologicalcharge.c:112-121)
-template <typename T>
-__device__ __forceinline__ Q<T> anti_hermitian(Q<T> X)
-{
-    Q<T> Xd = qdag(X);
-    X = Q<T>{X.c0 - Xd.c0, X.c1 - Xd.c1, X.c2 - Xd.c2, X.c3 - Xd.c3};
-    return qdiv(X, T(2));
-}
-
-// A site walked along the two directions of one plane: its coordinates there,
-// its cb index and parity.  Each step updates them in place (compares and
-// selects, no multiplications); recomputing the cb index from the
-// coordinates at every one of the charge's 576 loads per site cost as many
-// instructions as the products (FP32 charge 7% slower).
-template <int D1, int D2> struct PlaneCursor {
-    int c[2];      // coordinates along D1, D2
-    uint32_t cb;   // cb index
-    int par;       // parity
-};
-
-template <int D1, int D2>
-__device__ __forceinline__ void pin(PlaneCursor<D1, D2>& n)
-{
-    asm volatile("" : "+v"(n.c[0]), "+v"(n.c[1]), "+v"(n.cb), "+v"(n.par));
-}
-
-// one step along +-D (D = D1 or D2) with periodic wrap
-template <int D1, int D2, int D>
-__device__ __forceinline__ void cstep(const Geo& g, PlaneCursor<D1, D2>& n, bool back)
-{
-    constexpr int k = (D == D1) ? 0 : 1;
-    const int L = g.L[D];
-    int c = n.c[k];
-    if (D == 3) {
-        // z: the cb index holds z/2
-        const uint32_t hz1 = (uint32_t)g.hz - 1u;
-        if (!back) {
-            n.cb += (c & 1) ? ((c + 1 == L) ? (uint32_t)0 - hz1 : 1u) : 0u;
-            c = (c + 1 == L) ? 0 : c + 1;
-        } else {
-            n.cb += (c & 1) ? 0u : ((c == 0) ? hz1 : (uint32_t)0 - 1u);
-            c = (c == 0) ? L - 1 : c - 1;
-        }
-    } else {
-        const uint32_t st = g.stride[D], wrap = (uint32_t)(L - 1) * st;
-        if (!back) {
-            n.cb += (c + 1 == L) ? (uint32_t)0 - wrap : st;
-            c = (c + 1 == L) ? 0 : c + 1;
-        } else {
-            n.cb += (c == 0) ? wrap : (uint32_t)0 - st;
-            c = (c == 0) ? L - 1 : c - 1;
-        }
-    }
-    n.c[k] = c;
-    n.par ^= 1;
-}
-
-template <typename T, int D1, int D2>
-__device__ __forceinline__ Q<T> clink(const Planes<T>& P, const Geo& g, int dir, const PlaneCursor<D1, D2>& n)
-{
-    return link_at(P, dir, ((uint32_t)n.par * g.vh + n.cb) * (uint32_t)sizeof(typename Store<T>::V));
-}
-
-// computeLine: l steps along +D (t = t U) or -D (t = t U^+), starting from
-// the identity when `first` (1*U = U exactly)
-template <typename T, int D1, int D2, int D>
-__device__ __forceinline__ void cwalk(const Planes<T>& P, const Geo& g, Q<T>& t, PlaneCursor<D1, D2>& n, int l,
-                                      bool back, bool& first)
-{
-    for (int i = 0; i < l; i++) {
-        if (!back) {
-            Q<T> u = clink(P, g, D, n);
-            if (first) t = u;
-            else t = qmul(t, u);
-            cstep<D1, D2, D>(g, n, false);
-        } else {
-            cstep<D1, D2, D>(g, n, true);
-            Q<T> u = clink(P, g, D, n);
-            if (first) t = qdag(u);
-            else t = qmul_dag(t, u);
-        }
-        first = false;
-    }
-}
-
-// computeCloverAverages(u, x, D1, D2, m, m): four m x m leaves / 4.  Each
-// leaf's product is complete before the next leaf's links are loaded: one
-// leaf (at most 12 links) live at a time keeps the charge kernel at several
-// waves per SIMD instead of one.
-template <typename T, int D1, int D2>
-__device__ __forceinline__ Q<T> clover_average(const Planes<T>& P, const Geo& g, PlaneCursor<D1, D2> n, int m)
-{
-    Q<T> sum, t;
-    bool f;
-    // +D1,+D2
-    f = true;
-    pin(n);
-    cwalk<T, D1, D2, D1>(P, g, t, n, m, false, f); cwalk<T, D1, D2, D2>(P, g, t, n, m, false, f);
-    cwalk<T, D1, D2, D1>(P, g, t, n, m, true, f);  cwalk<T, D1, D2, D2>(P, g, t, n, m, true, f);
-    sum = t;
-    pin(sum);
-    asm volatile("" ::: "memory");
-    // +D1,-D2
-    f = true;
-    pin(n);
-    cwalk<T, D1, D2, D2>(P, g, t, n, m, true, f);  cwalk<T, D1, D2, D1>(P, g, t, n, m, false, f);
-    cwalk<T, D1, D2, D2>(P, g, t, n, m, false, f); cwalk<T, D1, D2, D1>(P, g, t, n, m, true, f);
-    sum = qadd(sum, t);
-    pin(sum);
-    asm volatile("" ::: "memory");
-    // -D1,+D2
-    f = true;
-    pin(n);
-    cwalk<T, D1, D2, D2>(P, g, t, n, m, false, f); cwalk<T, D1, D2, D1>(P, g, t, n, m, true, f);
-    cwalk<T, D1, D2, D2>(P, g, t, n, m, true, f);  cwalk<T, D1, D2, D1>(P, g, t, n, m, false, f);
-    sum = qadd(sum, t);
-    pin(sum);
-    asm volatile("" ::: "memory");
-    // -D1,-D2
-    f = true;
-    pin(n);
-    cwalk<T, D1, D2, D1>(P, g, t, n, m, true, f);  cwalk<T, D1, D2, D2>(P, g, t, n, m, true, f);
-    cwalk<T, D1, D2, D1>(P, g, t, n, m, false, f); cwalk<T, D1, D2, D2>(P, g, t, n, m, false, f);
-    sum = qadd(sum, t);
-    pin(sum);
-    asm volatile("" ::: "memory");
-    return qdiv(sum, T(4));
-}
-
-// F_{D1 D2} = AH(1.5 C1 - 0.15 C2 + C3/90) at the site s (cb index i,
-// parity p), topologicalcharge.c:123-162
-template <typename T, int D1, int D2>
-__device__ __forceinline__ Q<T> improved_F(const Planes<T>& P, const Geo& g, const Site& s, uint32_t i, int p)
-{
-    const PlaneCursor<D1, D2> x{{s.c[D1], s.c[D2]}, i, p};
-    const T k1 = T(1.5), k2 = T(-0.15), k3 = T(1.) / T(90.);
-    Q<T> U1 = qscale(clover_average<T, D1, D2>(P, g, x, 1), k1);
-    Q<T> U2 = qscale(clover_average<T, D1, D2>(P, g, x, 2), k2);
-    Q<T> U3 = qscale(clover_average<T, D1, D2>(P, g, x, 3), k3);
-    Q<T> F = qadd(U1, U2);
-    F = qadd(F, U3);
-    return anti_hermitian(F);
-}
-
 // 1x1 leaves (topologicalcharge.c:201-290)
 template <typename T>
 __device__ __forceinline__ Q<T> U_plpl(const Planes<T>& P, const Geo& g, const Site& x, int jj, int kk)
@@ -1335,14 +1192,207 @@ __device__ __forceinline__ T clover_products_1x1(const Planes<T>& P, const Geo& 
     return T(2) * qmul(P0, P1).c0;
 }
 
-// KIND 0: improved (8 cp(0123) + 8 cp(0231) + 8 cp(0312)),
-// KIND 1: 1x1 clover (-cp(0132) - cp(1230) - cp(2031)).
-// grid: x over (tile, parity) pairs.  partial[k] = sum q and
-// partial[ncols + k] = sum |q| with k = p*tiles + tile, ncols = 2*tiles;
-// qx: density in lexicographic order (nullable).
-template <typename T, int KIND>
-__global__ void __launch_bounds__(BLOCK) k_qtop(Planes<T> P, Geo g, DD* __restrict__ partial,
-                                                double* __restrict__ qx)
+// ---------------------------------------------------------------------------
+// The improved clover from straight-line fields.  A leaf of side m is four
+// straight lines of m links, Lm_mu(y) = U_mu(y) U_mu(y+mu) ... U_mu(y+(m-1)mu):
+// three products of four loaded lines instead of 4m - 1 products of 4m
+// loaded links, and the four leaves of one (plane, m) share their lines: 12
+// loads, 216 per site for the six F, against 576 link loads walking the
+// leaves link by link.  The lines of two and three links are built once per
+// charge (k_qlines).  The reference multiplies each leaf's 4m links
+// sequentially (computeLine, topologicalcharge.c:18-44); grouping them by
+// lines changes the rounding of q(x) by a few units in the last place.
+// ---------------------------------------------------------------------------
+// L2_mu(x) = U_mu(x) U_mu(x+mu), L3_mu(x) = L2_mu(x) U_mu(x+2mu).  One thread
+// per pair of sites x, x+mu with x_mu even: the four links U_mu(x .. x+3mu)
+// serve the lines of both sites (two loads per site instead of three).
+// Threads run over the lattice halved along mu, z fastest, so a wave covers
+// consecutive sites of one row.  grid: x over VOL/2 threads, y = mu
+template <typename T>
+__global__ void __launch_bounds__(BLOCK) k_qlines(Planes<T> U, Planes<T> L2, Planes<T> L3, Geo g)
+{
+    const uint32_t j = blockIdx.x * BLOCK + threadIdx.x;
+    const int mu = blockIdx.y;
+    if (j >= g.vh) return;
+    int h[4];
+    for (int d = 0; d < 4; d++) h[d] = (d == mu) ? g.L[d] / 2 : g.L[d];
+    Site s;
+    uint32_t r = j;
+    s.c[3] = (int)(r % (uint32_t)h[3]);
+    r /= (uint32_t)h[3];
+    s.c[2] = (int)(r % (uint32_t)h[2]);
+    r /= (uint32_t)h[2];
+    s.c[1] = (int)(r % (uint32_t)h[1]);
+    s.c[0] = (int)(r / (uint32_t)h[1]);
+    s.c[mu] *= 2;
+    const Site s1 = shift(g, s, mu, 1), s2 = shift(g, s1, mu, 1), s3 = shift(g, s2, mu, 1);
+    const Q<T> u0 = link(U, g, mu, s), u1 = link(U, g, mu, s1), u2 = link(U, g, mu, s2), u3 = link(U, g, mu, s3);
+    const int p = parity(s);
+    const uint32_t i0 = cb_index(g, s), i1 = cb_index(g, s1);
+    const Q<T> a = qmul(u0, u1), b = qmul(u1, u2);
+    qstore<T>(L2.p[mu][p], i0, a);
+    qstore<T>(L3.p[mu][p], i0, qmul(a, u2));
+    qstore<T>(L2.p[mu][p ^ 1], i1, b);
+    qstore<T>(L3.p[mu][p ^ 1], i1, qmul(b, u3));
+}
+
+// the part of the cb index that coordinate c along D contributes (z: the cb
+// index holds z/2)
+template <int D>
+__device__ __forceinline__ uint32_t cb_part(const Geo& g, int c)
+{
+    return (D == 3) ? (uint32_t)(c >> 1) : (uint32_t)c * g.stride[D];
+}
+
+// c mod L[D] for -2 L < c < 2 L: two wraps, since a shift of m <= 3 links
+// exceeds one period on lattices of extent 2
+template <int D>
+__device__ __forceinline__ int wrap_coord(const Geo& g, int c)
+{
+#pragma unroll
+    for (int k = 0; k < 2; k++) {
+        c = (c >= g.L[D]) ? c - g.L[D] : c;
+        c = (c < 0) ? c + g.L[D] : c;
+    }
+    return c;
+}
+
+// The vector part (c1, c2, c3) of a quaternion: all that the field strength
+// keeps.  AH(X) = (X - X^+)/2 (topologicalcharge.c:112-121) zeroes c0 and
+// returns the vector part of X exactly ((c + c)/2 = c), so the clover sums,
+// their scalings and the last product of each leaf form only c1..c3, each
+// with the reference's operations, and Re tr(F G) = 2 (F G)_0 = -2 f.g.
+template <typename T> struct V3 { T c1, c2, c3; };
+
+// vector part of v w (gauge.h:83-91)
+template <typename T>
+__device__ __forceinline__ V3<T> vmul(const Q<T>& v, const Q<T>& w)
+{
+    return V3<T>{v.c0 * w.c1 + v.c1 * w.c0 - v.c2 * w.c3 + v.c3 * w.c2,
+                 v.c0 * w.c2 + v.c1 * w.c3 + v.c2 * w.c0 - v.c3 * w.c1,
+                 v.c0 * w.c3 - v.c1 * w.c2 + v.c2 * w.c1 + v.c3 * w.c0};
+}
+
+// vector part of v w^+ (gauge.h:94-102)
+template <typename T>
+__device__ __forceinline__ V3<T> vmul_dag(const Q<T>& v, const Q<T>& w)
+{
+    return V3<T>{-v.c0 * w.c1 + v.c1 * w.c0 + v.c2 * w.c3 - v.c3 * w.c2,
+                 -v.c0 * w.c2 - v.c1 * w.c3 + v.c2 * w.c0 + v.c3 * w.c1,
+                 -v.c0 * w.c3 + v.c1 * w.c2 - v.c2 * w.c1 + v.c3 * w.c0};
+}
+
+template <typename T>
+__device__ __forceinline__ V3<T> vadd(const V3<T>& a, const V3<T>& b) { return V3<T>{a.c1 + b.c1, a.c2 + b.c2, a.c3 + b.c3}; }
+
+template <typename T>
+__device__ __forceinline__ void pin(V3<T>& v)
+{
+    asm volatile("" : "+v"(v.c1), "+v"(v.c2), "+v"(v.c3));
+}
+
+// Index offsets of a site x along each direction d: moving by a*m links along
+// d (a = -1, 0, +1) adds delta[d][a+1] to the cb index (0 for a = 0); the
+// parity changes iff m is odd.
+struct LineGeo {
+    int32_t delta[4][3];
+};
+
+template <int D>
+__device__ __forceinline__ void line_offsets(const Geo& g, const Site& s, int m, LineGeo& lg)
+{
+    const int c = s.c[D];
+    const uint32_t here = cb_part<D>(g, c);
+    lg.delta[D][0] = (int32_t)(cb_part<D>(g, wrap_coord<D>(g, c - m)) - here);
+    lg.delta[D][1] = 0;
+    lg.delta[D][2] = (int32_t)(cb_part<D>(g, wrap_coord<D>(g, c + m)) - here);
+}
+
+// the line along `dir` at x + (ia-1) m D1 + (ib-1) m D2
+template <typename T, int D1, int D2>
+__device__ __forceinline__ Q<T> line_at(const Planes<T>& Lm, const Geo& g, const LineGeo& lg, uint32_t i, int p,
+                                        int odd, int dir, int ia, int ib)
+{
+    const uint32_t par = (uint32_t)(p ^ ((ia + ib) & odd));
+    const uint32_t cb = i + (uint32_t)lg.delta[D1][ia] + (uint32_t)lg.delta[D2][ib];
+    return link_at(Lm, dir, (par * g.vh + cb) * (uint32_t)sizeof(typename Store<T>::V));
+}
+
+// computeCloverAverages(u, x, D1, D2, m, m) (topologicalcharge.c:46-110) from
+// the lines Lm of side m (m = 1: the links): the four leaves in the
+// reference's order and orientation, summed in its order, / 4 (vector part).
+// S[d] = Lm_d(x) and B[d] = Lm_d(x - m d), which every plane containing d
+// uses, are loaded once per site and m by the caller.
+template <typename T, int D1, int D2>
+__device__ __forceinline__ V3<T> clover_lines(const Planes<T>& Lm, const Geo& g, const LineGeo& lg, uint32_t i,
+                                              int p, int odd, const Q<T>* S, const Q<T>* B)
+{
+    auto ld = [&](int dir, int ia, int ib) { return line_at<T, D1, D2>(Lm, g, lg, i, p, odd, dir, ia, ib); };
+    Q<T> t;
+    V3<T> sum;
+    // +D1,+D2: L1(x) L2(x+m D1) L1(x+m D2)^+ L2(x)^+
+    t = qmul(S[D1], ld(D2, 2, 1));
+    t = qmul_dag(t, ld(D1, 1, 2));
+    sum = vmul_dag(t, S[D2]);
+    // +D1,-D2: -D2, +D1, +D2, -D1
+    t = qmul(qdag(B[D2]), ld(D1, 1, 0));
+    t = qmul(t, ld(D2, 2, 0));
+    sum = vadd(sum, vmul_dag(t, S[D1]));
+    // -D1,+D2: +D2, -D1, -D2, +D1
+    t = qmul_dag(S[D2], ld(D1, 0, 2));
+    t = qmul_dag(t, ld(D2, 0, 1));
+    sum = vadd(sum, vmul(t, B[D1]));
+    // -D1,-D2: -D1, -D2, +D1, +D2
+    t = qmul_dag(qdag(B[D1]), ld(D2, 0, 0));
+    t = qmul(t, ld(D1, 0, 0));
+    sum = vadd(sum, vmul(t, B[D2]));
+    return V3<T>{sum.c1 / T(4), sum.c2 / T(4), sum.c3 / T(4)};
+}
+
+// F += k C_m for the six planes of the charge, in the order (0,1), (2,3),
+// (0,2), (3,1), (0,3), (1,2); m = 1 starts the sums (F_{D1 D2} =
+// AH(1.5 C1 - 0.15 C2 + C3/90), topologicalcharge.c:123-162, summed in its order)
+template <typename T>
+__device__ __forceinline__ void add_cloverlines(const Planes<T>& Lm, const Geo& g, const Site& s, uint32_t i, int p,
+                                                int m, T k, V3<T>* F)
+{
+    LineGeo lg;
+    line_offsets<0>(g, s, m, lg);
+    line_offsets<1>(g, s, m, lg);
+    line_offsets<2>(g, s, m, lg);
+    line_offsets<3>(g, s, m, lg);
+    const int odd = m & 1;
+    const uint32_t sz = (uint32_t)sizeof(typename Store<T>::V);
+    Q<T> S[4], B[4];
+#pragma unroll
+    for (int d = 0; d < 4; d++) {
+        S[d] = link_at(Lm, d, ((uint32_t)p * g.vh + i) * sz);
+        B[d] = link_at(Lm, d, ((uint32_t)(p ^ odd) * g.vh + i + (uint32_t)lg.delta[d][0]) * sz);
+    }
+    auto acc = [&](int f, const V3<T>& C) {
+        const V3<T> u{C.c1 * k, C.c2 * k, C.c3 * k};
+        F[f] = (m == 1) ? u : vadd(F[f], u);
+        pin(F[f]);
+    };
+    acc(0, clover_lines<T, 0, 1>(Lm, g, lg, i, p, odd, S, B));
+    acc(1, clover_lines<T, 2, 3>(Lm, g, lg, i, p, odd, S, B));
+    acc(2, clover_lines<T, 0, 2>(Lm, g, lg, i, p, odd, S, B));
+    acc(3, clover_lines<T, 3, 1>(Lm, g, lg, i, p, odd, S, B));
+    acc(4, clover_lines<T, 0, 3>(Lm, g, lg, i, p, odd, S, B));
+    acc(5, clover_lines<T, 1, 2>(Lm, g, lg, i, p, odd, S, B));
+}
+
+// 2 (F G)_0 of two field strengths with c0 = 0 (gauge.h:83-91, 177-178)
+template <typename T>
+__device__ __forceinline__ T tr_FF(const V3<T>& f, const V3<T>& g)
+{
+    return T(2) * (-f.c1 * g.c1 - f.c2 * g.c2 - f.c3 * g.c3);
+}
+
+// the improved charge density from the line fields; grid and partials as k_qtop
+template <typename T>
+__global__ void __launch_bounds__(BLOCK) k_qtop_lines(Planes<T> P, Planes<T> L2, Planes<T> L3, Geo g,
+                                                      DD* __restrict__ partial, double* __restrict__ qx)
 {
     int p;
     uint32_t base = tile_base_pair(g, p);
@@ -1351,14 +1401,44 @@ __global__ void __launch_bounds__(BLOCK) k_qtop(Planes<T> P, Geo g, DD* __restri
     double q = 0.0;
     if (i < g.vh) {
         Site s = cb_site(g, i, p);
-        T v;
-        if (KIND == 0)
-            v = T(8.) * (T(2) * qmul(improved_F<T, 0, 1>(P, g, s, i, p), improved_F<T, 2, 3>(P, g, s, i, p)).c0) +
-                T(8.) * (T(2) * qmul(improved_F<T, 0, 2>(P, g, s, i, p), improved_F<T, 3, 1>(P, g, s, i, p)).c0) +
-                T(8.) * (T(2) * qmul(improved_F<T, 0, 3>(P, g, s, i, p), improved_F<T, 1, 2>(P, g, s, i, p)).c0);
-        else
-            v = -clover_products_1x1(P, g, s, 0, 1, 3, 2) - clover_products_1x1(P, g, s, 1, 2, 3, 0) -
-                clover_products_1x1(P, g, s, 2, 0, 3, 1);
+        V3<T> F[6];
+        add_cloverlines(P, g, s, i, p, 1, T(1.5), F);
+        pin(s);
+        add_cloverlines(L2, g, s, i, p, 2, T(-0.15), F);
+        pin(s);
+        add_cloverlines(L3, g, s, i, p, 3, T(1.) / T(90.), F);
+        T v = T(8.) * tr_FF(F[0], F[1]);
+        v = v + T(8.) * tr_FF(F[2], F[3]);
+        v = v + T(8.) * tr_FF(F[4], F[5]);
+        q = (double)v;
+        if (qx) qx[lex_index(g, s)] = q;
+    }
+    DD a = block_sum_dd(DD{q, 0.0});
+    DD b = block_sum_dd(DD{fabs(q), 0.0});
+    if (threadIdx.x == 0) {
+        size_t k = (size_t)p * (gridDim.x >> 1) + tile;
+        partial[k] = a;
+        partial[(size_t)gridDim.x + k] = b;
+    }
+}
+
+// the 1x1 clover charge density, -cp(0132) - cp(1230) - cp(2031)
+// (topologicalcharge.c:391-409).  grid: x over (tile, parity) pairs.
+// partial[k] = sum q and partial[ncols + k] = sum |q| with k = p*tiles + tile,
+// ncols = 2*tiles; qx: density in lexicographic order (nullable).
+template <typename T>
+__global__ void __launch_bounds__(BLOCK) k_qtop_clover(Planes<T> P, Geo g, DD* __restrict__ partial,
+                                                       double* __restrict__ qx)
+{
+    int p;
+    uint32_t base = tile_base_pair(g, p);
+    uint32_t tile = base / BLOCK;
+    uint32_t i = base + threadIdx.x;
+    double q = 0.0;
+    if (i < g.vh) {
+        Site s = cb_site(g, i, p);
+        const T v = -clover_products_1x1(P, g, s, 0, 1, 3, 2) - clover_products_1x1(P, g, s, 1, 2, 3, 0) -
+                    clover_products_1x1(P, g, s, 2, 0, 3, 1);
         q = (double)v;
         if (qx) qx[lex_index(g, s)] = q;
     }

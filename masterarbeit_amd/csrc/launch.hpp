@@ -43,9 +43,12 @@ template <typename T> struct Launch {
     static hipError_t sline(hipStream_t s, const Geo& g, const RawPlanes& P, const RawPlanes& S, int R0, int R1);
     static hipError_t wloops(hipStream_t s, const Geo& g, const RawPlanes& P, const RawPlanes& S,
                              const WGroupHost* groups, int group, const int* slots, void* partial, int ncols);
-    // partial: 2 rows of 2*ceil(vh/256) columns (sum q, sum |q|)
-    static hipError_t qtop(hipStream_t s, const Geo& g, const RawPlanes& P, int kind, void* partial,
-                           double* density);
+    // partial: 2 rows of 2*ceil(vh/256) columns (sum q, sum |q|); the improved
+    // charge (kind 0) reads the line fields L2, L3 that qlines builds from P
+    static hipError_t qlines(hipStream_t s, const Geo& g, const RawPlanes& P, const RawPlanes& L2,
+                             const RawPlanes& L3);
+    static hipError_t qtop(hipStream_t s, const Geo& g, const RawPlanes& P, int kind, const RawPlanes& L2,
+                           const RawPlanes& L3, void* partial, double* density);
     // adjoint of APE, mode 0 spatial / 1 all / 2 temporal (vjp_kernels.hpp)
     static hipError_t ape_save(hipStream_t s, const Geo& g, const RawPlanes& in, const RawPlanes& out,
                                const RawNorms& mn, int mode, double sp);

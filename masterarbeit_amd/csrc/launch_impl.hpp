@@ -185,12 +185,23 @@ hipError_t Launch<T>::wloops(hipStream_t s, const Geo& g, const RawPlanes& P, co
 }
 
 template <typename T>
-hipError_t Launch<T>::qtop(hipStream_t s, const Geo& g, const RawPlanes& P, int kind, void* partial,
-                           double* density)
+hipError_t Launch<T>::qlines(hipStream_t s, const Geo& g, const RawPlanes& P, const RawPlanes& L2,
+                             const RawPlanes& L3)
+{
+    hipLaunchKernelGGL(k_qlines<T>, dim3(grid_x(g), 4), dim3(BLOCK), 0, s, planes<T>(P), planes<T>(L2), planes<T>(L3),
+                       g);
+    return hipGetLastError();
+}
+
+template <typename T>
+hipError_t Launch<T>::qtop(hipStream_t s, const Geo& g, const RawPlanes& P, int kind, const RawPlanes& L2,
+                           const RawPlanes& L3, void* partial, double* density)
 {
     dim3 grid(2 * grid_x(g)), block(BLOCK);
-    if (kind == 0) hipLaunchKernelGGL((k_qtop<T, 0>), grid, block, 0, s, planes<T>(P), g, (DD*)partial, density);
-    else hipLaunchKernelGGL((k_qtop<T, 1>), grid, block, 0, s, planes<T>(P), g, (DD*)partial, density);
+    if (kind == 0)
+        hipLaunchKernelGGL(k_qtop_lines<T>, grid, block, 0, s, planes<T>(P), planes<T>(L2), planes<T>(L3), g,
+                           (DD*)partial, density);
+    else hipLaunchKernelGGL(k_qtop_clover<T>, grid, block, 0, s, planes<T>(P), g, (DD*)partial, density);
     return hipGetLastError();
 }
 

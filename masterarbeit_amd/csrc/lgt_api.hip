@@ -118,6 +118,9 @@ struct lgt_engine {
     size_t link_buf_bytes = 0;
     void* uni_buf = nullptr;    // per-link heatbath: the injected uniforms
     size_t uni_buf_bytes = 0;
+    void* qline = nullptr;      // improved charge: lines of 2 and 3 links (two fields)
+    size_t qline_bytes = 0;
+    RawPlanes qL2, qL3;
     int profile = 0;
     int order = -1;   // tile order of every kernel; -1: per kernel (see kgeo)
     int group = 0;    // LGT_OPT_GROUP (0: per kernel, see kgeo)
@@ -594,6 +597,7 @@ int lgt_destroy(lgt_engine* e)
     if (e->staging) hipFree(e->staging);
     if (e->link_buf) hipFree(e->link_buf);
     if (e->uni_buf) hipFree(e->uni_buf);
+    if (e->qline) hipFree(e->qline);
     if (e->hres) hipHostFree(e->hres);
     for (int b = 0; b < 2; b++) {
         if (e->pin[b]) hipHostFree(e->pin[b]);
@@ -1148,8 +1152,19 @@ int lgt_qtop(lgt_engine* e, int kind, double* q, double* abs_sum, double* densit
         TRY(ensure(e, &e->staging, &e->staging_bytes, sizeof(double) * (size_t)e->vol, "staging"));
         dq = (double*)e->staging;
     }
-    TRY(run(e, LGT_K_QTOP, "qtop",
-            [&] { return DISPATCH(e, qtop(e->stream, kgeo(e, LGT_K_QTOP, 0), e->cur, kind, e->partial, dq)); }));
+    if (kind == LGT_Q_IMPROVED) {
+        const size_t fb = (size_t)4 * e->vol * e->link_bytes;
+        if (!e->qline) {
+            TRY(ensure(e, &e->qline, &e->qline_bytes, 2 * fb, "charge line fields"));
+            set_planes(e->qL2, e->qline, e->geo.vh, e->link_bytes);
+            set_planes(e->qL3, (char*)e->qline + fb, e->geo.vh, e->link_bytes);
+        }
+        TRY(run(e, LGT_K_QTOP, "charge lines",
+                [&] { return DISPATCH(e, qlines(e->stream, e->geo, e->cur, e->qL2, e->qL3)); }));
+    }
+    TRY(run(e, LGT_K_QTOP, "qtop", [&] {
+        return DISPATCH(e, qtop(e->stream, kgeo(e, LGT_K_QTOP, 0), e->cur, kind, e->qL2, e->qL3, e->partial, dq));
+    }));
     double s[2];
     TRY(reduce_to_host(e, 2, (int)(2 * nb), s));
     const double PI = 3.14159265358979323846;
