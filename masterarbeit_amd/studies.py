@@ -2,7 +2,7 @@
 """Physics studies on the GPU engine: thermalisation, FP32-storage precision,
 topology and a 96^4 production chain.
 
-    python -m masterarbeit_amd.studies c1|c2|c3|c4|c5|c6|c7|all [--quick]
+    python -m masterarbeit_amd.studies c1|c2|c3|c4|c5|c6|c7|c8|all [--quick]
 
 Each study prints one JSON line on stdout (progress on stderr).  The field is
 resident on the device throughout; host transfers happen only where a
@@ -33,6 +33,10 @@ and FP64 storage) or restored after a destructive measurement.
       chains (4 seeds, hot and cold starts, 3000 thermalisation sweeps, 1000
       measurements 4 sweeps apart) against the reference's own chain at
       beta 3.09 (W_REF).
+  c8  24^4: W(R,T) for (1,1), (3,3), (5,5), (7,7), (9,9), (10,16) of FP64 and FP32-storage
+      chains (cold and hot starts, 2000 thermalisation sweeps, --nmeas measurements --every
+      sweeps apart) and W(10,16) after 100 spatial APE steps, against the reference's own
+      equilibrated 24^4 chains (W_REF24).
 """
 import argparse
 import json
