@@ -334,6 +334,9 @@ def cpu_baseline(U, L, beta, procs):
     t0 = time.perf_counter()
     ref.lib.ref_update(1)
     dt = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    ref.lib.ref_ape(0, 1, 0.25)     # one spatial APE step (smearing.c:107-138) of the same field
+    dt_ape = time.perf_counter() - t0
     out = {
         "value": 4.0 * vol / dt,
         "unit": "link updates/s",
@@ -341,6 +344,7 @@ def cpu_baseline(U, L, beta, procs):
         "kind": "reference",
         "sample": f"1 lexicographic heatbath sweep (gaugefieldUpdate, no overrelaxation) of the thermalised "
                   f"{'x'.join(map(str, L))} FP64 configuration, {dt:.1f} s on 1 host core",
+        "spatial_ape_step_s": dt_ape,
     }
     if procs > 1:
         with tempfile.TemporaryDirectory() as tmp:
