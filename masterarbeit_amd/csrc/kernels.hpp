@@ -984,7 +984,8 @@ __global__ void __launch_bounds__(BLOCK) k_ape_zpair(Planes<T> Z, Planes<T> O, G
 // wilson.c:21-56,268-324), fused over all T of one R:
 //   W(x) = Re tr[ L_T(x) S_R(x+T t) L_T(x+R d)^+ S_R(x)^+ ]
 //        = 2 < L_T(x) S_R(x+T t), S_R(x) L_T(x+R d) >
-// with temporal lines L_T grown one link per step and the spatial lines S_R
+// with the temporal line L_T(x) and the product S_R(x) L_T(x+R d) each grown
+// by one link per step (three products per T), and the spatial lines S_R
 // read from a line field built beforehand (k_sline), one load per step
 // instead of R.  Results per (group, d, parity, tile) are reduced in
 // double-double; every wave adds its shuffle-reduced value to its own LDS slot.
@@ -1028,8 +1029,9 @@ __global__ void __launch_bounds__(BLOCK) k_sline(Planes<T> P, Planes<T> S, Geo g
 // sites of it (stride THREADS): the lane sums its sites' values per T in
 // double, so one wave-wide double-double sum per T serves SITES sites.  FP32
 // takes 4 sites per lane (one wave per tile), FP64, whose lines need twice the
-// registers, 2 (measured at 48^4, 160 loops: FP32 17.1 -> 11.4 ms with 4 sites
-// per lane, FP64 30.1 -> 27.1 ms with 2, 29.6 ms with 4).
+// registers, 2 (measured at 48^4, 160 loops with the line fields: FP32 9.45 ms
+// with 4 sites per lane, 10.8 ms with 2; FP64 19.1 ms with 2, 19.5 ms with 4,
+// 23.0 ms with 1).
 // grid: x over (tile, parity) pairs, y = d - 1; WL<T>::THREADS per block
 template <typename T> struct WL {
     static constexpr int SITES = (sizeof(T) == 4) ? 4 : 2;
@@ -1056,7 +1058,7 @@ __global__ void __launch_bounds__(WL<T>::THREADS) k_wloops(Planes<T> P, Planes<T
     // index, advanced one time step per T (t0: their time offset, mod L0)
     int tT[NS], tR[NS];
     uint32_t cbT[NS], cbR[NS];
-    Q<T> SR[NS], Lx[NS], LxR[NS];
+    Q<T> Lx[NS], B[NS];   // L_T(x) and S_R(x) L_T(x + R d)
     bool valid[NS];
     const int pR = p ^ (R & 1);   // parity of x + R d
     const uint32_t st0 = g.stride[0], wrap0 = (uint32_t)(g.L[0] - 1) * st0;
@@ -1071,9 +1073,8 @@ __global__ void __launch_bounds__(WL<T>::THREADS) k_wloops(Planes<T> P, Planes<T
         cbT[j] = valid[j] ? i : 0u;
         tR[j] = sR.c[0];
         cbR[j] = cb_index(g, sR);
-        SR[j] = link(S, g, d, s0);
         Lx[j] = link(P, g, 0, s0);
-        LxR[j] = link(P, g, 0, sR);
+        B[j] = qmul(link(S, g, d, s0), link(P, g, 0, sR));
     }
     int t0 = 0;
     for (int Tt = 1; Tt <= grp.Tmax; Tt++) {
@@ -1084,7 +1085,7 @@ __global__ void __launch_bounds__(WL<T>::THREADS) k_wloops(Planes<T> P, Planes<T
         for (int j = 0; j < NS; j++) {
             if (Tt > 1) {
                 Lx[j] = qmul(Lx[j], link_at(P, 0, (((uint32_t)p ^ odd) * g.vh + cbT[j]) * LB));
-                LxR[j] = qmul(LxR[j], link_at(P, 0, (((uint32_t)pR ^ odd) * g.vh + cbR[j]) * LB));
+                B[j] = qmul(B[j], link_at(P, 0, (((uint32_t)pR ^ odd) * g.vh + cbR[j]) * LB));
             }
             // one step in t
             cbT[j] += (tT[j] + 1 == g.L[0]) ? (uint32_t)0 - wrap0 : st0;
@@ -1093,7 +1094,7 @@ __global__ void __launch_bounds__(WL<T>::THREADS) k_wloops(Planes<T> P, Planes<T
             tR[j] = (tR[j] + 1 == g.L[0]) ? 0 : tR[j] + 1;
             if (slot >= 0) {
                 const Q<T> St = link_at(S, d, (((uint32_t)p ^ odd ^ 1u) * g.vh + cbT[j]) * LB);
-                const T w = T(2) * qdot(qmul(Lx[j], St), qmul(SR[j], LxR[j]));
+                const T w = T(2) * qdot(qmul(Lx[j], St), B[j]);
                 acc += valid[j] ? (double)w : 0.0;
             }
         }
