@@ -10,7 +10,7 @@ template struct Launch<double>;
 
 hipError_t launch_reduce_rows(hipStream_t s, const void* in, int nrows, int ncols, void* out)
 {
-    hipLaunchKernelGGL(k_reduce_rows, dim3(nrows), dim3(BLOCK), 0, s, (const DD*)in, ncols, (DD*)out);
+    hipLaunchKernelGGL(k_reduce_rows, dim3(nrows), dim3(REDUCE_THREADS), 0, s, (const DD*)in, ncols, (DD*)out);
     return hipGetLastError();
 }
 

@@ -79,7 +79,7 @@ extern "C" int lgt_plaquette_aos_noalloc(const int extents[4], const double* fie
     const int nb = (int)((vol + BLOCK - 1) / BLOCK);
     hipStream_t s = (hipStream_t)stream;
     hipLaunchKernelGGL(k_plaquette_aos, dim3(nb), dim3(BLOCK), 0, s, field_dev, g, sc);
-    hipLaunchKernelGGL(k_reduce_rows, dim3(1), dim3(BLOCK), 0, s, (const DD*)sc, nb, sc + NOALLOC_PARTIALS);
+    hipLaunchKernelGGL(k_reduce_rows, dim3(1), dim3(REDUCE_THREADS), 0, s, (const DD*)sc, nb, sc + NOALLOC_PARTIALS);
     hipLaunchKernelGGL(k_plaquette_mean, dim3(1), dim3(64), 0, s, (const DD*)(sc + NOALLOC_PARTIALS),
                        (double)(2 * 6 * vol), plaq_dev);
     err = hipGetLastError();
