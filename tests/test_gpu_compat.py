@@ -176,6 +176,26 @@ def test_local_heatbath_degenerate_staple():
     assert np.array_equal(G, U)     # nothing else changed
 
 
+def test_per_link_entry_points_reject_bad_links():
+    """lgt_staple and lgt_local_heatbath return LGT_EINVAL for a site outside
+    the lattice, a direction outside 0..3 and a negative or non-finite beta,
+    and leave the field as it was."""
+    import masterarbeit_amd as lgt
+    lgt.load_library()
+    with lgt.Engine((4, 4, 4, 4), lgt.FP64, 3) as e:
+        e.init_field(1)
+        U = e.download()
+        for site, mu in [(-1, 0), (256, 0), (0, -1), (0, 4)]:
+            with pytest.raises(lgt.LgtError, match="outside the lattice"):
+                e.staple(site, mu)
+            with pytest.raises(lgt.LgtError, match="outside the lattice"):
+                e.local_heatbath(site, mu, 3.09, [0.5] * 6)
+        for beta in (-1.0, float("nan"), float("inf")):
+            with pytest.raises(lgt.LgtError, match="beta"):
+                e.local_heatbath(5, 1, beta, [0.5] * 6)
+        assert np.array_equal(e.download(), U)
+
+
 QCD = os.path.join(ROOT, "oracle", "_ref", "qcd_l8", "qcd")
 
 

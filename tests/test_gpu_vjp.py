@@ -53,10 +53,13 @@ MODES = [(0, 0.25), (1, 1.0 / 6.0), (2, 0.25)]     # (mode, sp): spatial, all di
 
 @pytest.mark.parametrize("mode,sp", MODES)
 @pytest.mark.parametrize("L,nsteps,segment", [((6, 4, 4, 8), 7, 0), ((6, 4, 4, 8), 7, 3), ((8, 8, 8, 8), 5, 5),
-                                              ((8, 8, 8, 8), 9, 2)])
+                                              ((8, 8, 8, 8), 9, 2), ((2, 2, 2, 4), 3, 0), ((4, 2, 6, 4), 4, 2)])
 def test_vjp_matches_cpu_chain_rule(lgt, L, nsteps, segment, mode, sp):
+    """Extents 2 (x + 1 and x - 1 the same site, loops of R = 2 closing on
+    themselves; L0 = 2 allows T = 1 only) and lattices smaller than one
+    256-site tile included."""
     o, U = thermal(L)
-    R, T = 2, 2
+    R, T = 2, min(2, L[0] - 1)
     Lref, Gref = o.ape_vjp(U, nsteps, sp, R, T, mode)
     with lgt.Engine(L, 0, 1) as e:
         e.upload(U)
