@@ -71,3 +71,21 @@ def test_engine_rejects_bad_arguments_without_device():
     ok = (C.c_int * 4)(4, 4, 4, 4)
     assert lib.lgt_create(ok, 7, C.c_uint64(1), C.byref(h)) == -1
     assert lib.lgt_strerror(-3) == b"HIP device error"
+    short_z = (C.c_int * 4)(4, 4, 4, 2)      # z pairs: L3 >= 4
+    assert lib.lgt_create(short_z, 0, C.c_uint64(1), C.byref(h)) == -1
+    assert b"extent[3]" in lib.lgt_last_error()
+    # both parity planes of a direction within 32-bit byte offsets: 108^4
+    # FP64 links (32 B) exceed them
+    big = (C.c_int * 4)(108, 108, 108, 108)
+    assert lib.lgt_create(big, 0, C.c_uint64(1), C.byref(h)) == -1
+    assert b"too large for FP64" in lib.lgt_last_error()
+    assert not h.value
+    # a null engine is an argument error in every entry point, not a crash
+    p = C.c_double(0.0)
+    assert lib.lgt_plaquette(None, C.byref(p)) == -1
+    assert lib.lgt_update(None, 1, 1, 3.09, None) == -1
+    assert lib.lgt_ape(None, 0, 1, 0.25) == -1
+    # the allocation-free plaquette checks its sizes before any device call
+    assert lib.lgt_plaquette_aos_noalloc((C.c_int * 4)(98, 96, 96, 96), C.c_void_p(16), C.c_void_p(16), None) == -1
+    assert b"scratch limit" in lib.lgt_last_error()
+    assert lib.lgt_plaquette_aos_noalloc((C.c_int * 4)(4, 4, 4, 4), None, C.c_void_p(16), None) == -1
