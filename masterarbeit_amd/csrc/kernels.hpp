@@ -229,9 +229,12 @@ __global__ void __launch_bounds__(BLOCK) k_from_aos(Planes<T> P, Geo g, const do
     d2v v[8];
 #pragma unroll
     for (int k = 0; k < 8; k++) v[k] = a[k];
+    // both parity planes from p[mu][0] (adjacent planes): a plane pointer
+    // chosen by the lane's parity would be read from the kernel arguments with
+    // a vector load, and every store would wait on the loads in flight
 #pragma unroll
     for (int mu = 0; mu < 4; mu++)
-        qstore<T>(P.p[mu][p], n >> 1,
+        qstore<T>(P.p[mu][0], (uint32_t)p * g.vh + (n >> 1),
                   Q<T>{(T)v[2 * mu].x, (T)v[2 * mu].y, (T)v[2 * mu + 1].x, (T)v[2 * mu + 1].y});
 }
 
@@ -245,7 +248,7 @@ __global__ void __launch_bounds__(BLOCK) k_to_aos(Planes<T> P, Geo g, double* __
     d2v* a = (d2v*)(aos + 16 * (size_t)n);
 #pragma unroll
     for (int mu = 0; mu < 4; mu++) {
-        const Q<T> q = qload<T>(P.p[mu][p], n >> 1);
+        const Q<T> q = qload<T>(P.p[mu][0], (uint32_t)p * g.vh + (n >> 1));   // adjacent parity planes
         d2v lo, hi;
         lo.x = (double)q.c0; lo.y = (double)q.c1;
         hi.x = (double)q.c2; hi.y = (double)q.c3;
@@ -929,6 +932,10 @@ __global__ void __launch_bounds__(BLOCK) k_ape(Planes<T> Z, Planes<T> O, Geo g, 
     constexpr bool MIXED = sizeof(T) == 8 && MODE == 0;
     int p;
     uint32_t i = MIXED ? tile_site_mixed(g, p) : tile_base_pair(g, p) + threadIdx.x;
+    // p is the same for the whole wave; in a scalar register the output plane
+    // pointers O.p[mu][p] come from scalar loads (a per-lane index makes them
+    // vector loads, and the stores wait on every load in flight)
+    p = __builtin_amdgcn_readfirstlane(p);
     if (i >= g.vh) return;
     const Site s = cb_site(g, i, p);
     Site spn[4], smn[4];
@@ -1247,10 +1254,13 @@ __global__ void __launch_bounds__(BLOCK) k_qlines(Planes<T> U, Planes<T> L2, Pla
     const int p = parity(s);
     const uint32_t i0 = cb_index(g, s), i1 = cb_index(g, s1);
     const Q<T> a = qmul(u0, u1), b = qmul(u1, u2);
-    qstore<T>(L2.p[mu][p], i0, a);
-    qstore<T>(L3.p[mu][p], i0, qmul(a, u2));
-    qstore<T>(L2.p[mu][p ^ 1], i1, b);
-    qstore<T>(L3.p[mu][p ^ 1], i1, qmul(b, u3));
+    // both parity planes from p[mu][0] (adjacent planes), not a plane pointer
+    // chosen by the lane's parity (read from the kernel arguments per lane)
+    const uint32_t o0 = (uint32_t)p * g.vh + i0, o1 = (uint32_t)(p ^ 1) * g.vh + i1;
+    qstore<T>(L2.p[mu][0], o0, a);
+    qstore<T>(L3.p[mu][0], o0, qmul(a, u2));
+    qstore<T>(L2.p[mu][0], o1, b);
+    qstore<T>(L3.p[mu][0], o1, qmul(b, u3));
 }
 
 // the part of the cb index that coordinate c along D contributes (z: the cb
