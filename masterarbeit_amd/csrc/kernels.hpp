@@ -161,7 +161,7 @@ __global__ void __launch_bounds__(BLOCK) k_cold(Planes<T> P, Geo g)
 {
     uint32_t i = blockIdx.x * BLOCK + threadIdx.x;
     int mu = blockIdx.y >> 1, p = blockIdx.y & 1;
-    if (i < g.vh) qstore<T>(P.p[mu][p], i, qunit<T>());
+    if (i < g.vh) qstore_fill<T>(P.p[mu][p], i, qunit<T>());
 }
 
 // Haar-uniform links (Shoemake's method) from the link's Philox stream,
@@ -182,7 +182,7 @@ __global__ void __launch_bounds__(BLOCK) k_hot(Planes<T> P, Geo g, uint64_t seed
     q.c1 = (T)(s1 * sin(t1));
     q.c2 = (T)(s1 * cos(t1));
     q.c3 = (T)(s2 * sin(t2));
-    qstore<T>(P.p[mu][p], i, q);
+    qstore_fill<T>(P.p[mu][p], i, q);
 }
 
 // site of lexicographic index n (n = z + L3(y + L2(x + L1 t)))
