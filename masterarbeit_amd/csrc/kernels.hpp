@@ -1047,17 +1047,50 @@ __global__ void __launch_bounds__(BLOCK) k_sline(Planes<T> P, Planes<T> S, Geo g
     S.p[d][p][i] = v;   // read back by the loops right after: a cached store
 }
 
+// Wave sums of eight per-lane values at once.  Three exchange stages (lane
+// masks 32, 16, 8) each keep the half of a lane's values that its lane bit
+// chooses and add the partner's copy of that half; three more (4, 2, 1) add
+// the lanes that hold the same value.  Lane l ends with the double-double sum
+// over the wave of v[(l >> 3) & 7], in a fixed order.  One wave_sum_dd per
+// value costs six double-double additions and shuffles each; this costs ten
+// for all eight.
+__device__ __forceinline__ DD shfl_xor_dd(const DD& v, int m)
+{
+    return DD{__shfl_xor(v.hi, m), __shfl_xor(v.lo, m)};
+}
+
+__device__ __forceinline__ DD wave_sum8_dd(const double (&v)[8], int lane)
+{
+    const bool b5 = (lane & 32) != 0, b4 = (lane & 16) != 0, b3 = (lane & 8) != 0;
+    DD u[4];
+#pragma unroll
+    for (int k = 0; k < 4; k++) {
+        const double mine = b5 ? v[4 + k] : v[k];
+        const double other = __shfl_xor(b5 ? v[k] : v[4 + k], 32);
+        u[k] = dd_add(DD{mine, 0.0}, other);
+    }
+    DD w[2];
+#pragma unroll
+    for (int k = 0; k < 2; k++) {
+        const DD mine = b4 ? u[2 + k] : u[k];
+        w[k] = dd_add(mine, shfl_xor_dd(b4 ? u[k] : u[2 + k], 16));
+    }
+    DD x = dd_add(b3 ? w[1] : w[0], shfl_xor_dd(b3 ? w[0] : w[1], 8));
+#pragma unroll
+    for (int m = 4; m > 0; m >>= 1) x = dd_add(x, shfl_xor_dd(x, m));
+    return x;
+}
+
 // One group (one R, all its T) per launch; S holds the lines S_R.  A block
 // covers one 256-site tile with WL<T>::THREADS lanes, each taking WL<T>::SITES
 // sites of it (stride THREADS): the lane sums its sites' values per T in
-// double, so one wave-wide double-double sum per T serves SITES sites.  FP32
-// takes 4 sites per lane (one wave per tile), FP64, whose lines need twice the
-// registers, 2 (measured at 48^4, 160 loops with the line fields: FP32 9.45 ms
-// with 4 sites per lane, 10.8 ms with 2; FP64 19.1 ms with 2, 19.5 ms with 4,
-// 23.0 ms with 1).
+// double, and the wave sums of eight T at a time are taken together
+// (wave_sum8_dd).
 // grid: x over (tile, parity) pairs, y = d - 1; WL<T>::THREADS per block
 template <typename T> struct WL {
-    static constexpr int SITES = (sizeof(T) == 4) ? 4 : 2;
+    // sites per lane; 48^4, 160 loops with the batched wave sums:
+    // FP32 1, 2, 4 sites -> 9.5, 7.9, 8.5 ms; FP64 -> 16.2, 15.6, 17.7 ms
+    static constexpr int SITES = 2;
     static constexpr int THREADS = BLOCK / SITES;
     static constexpr int NW = THREADS / 64;
 };
@@ -1100,32 +1133,38 @@ __global__ void __launch_bounds__(WL<T>::THREADS) k_wloops(Planes<T> P, Planes<T
         B[j] = qmul(link(S, g, d, s0), link(P, g, 0, sR));
     }
     int t0 = 0;
-    for (int Tt = 1; Tt <= grp.Tmax; Tt++) {
-        const int slot = slots[grp.slot_off + Tt - 1];
-        const uint32_t odd = (uint32_t)(t0 & 1);
-        double acc = 0.0;
+    // eight T per batch; the lanes' sums of a batch are reduced together
+    for (int Tb = 1; Tb <= grp.Tmax; Tb += 8) {
+        double acc[8];
 #pragma unroll
-        for (int j = 0; j < NS; j++) {
-            if (Tt > 1) {
-                Lx[j] = qmul(Lx[j], link_at(P, 0, (((uint32_t)p ^ odd) * g.vh + cbT[j]) * LB));
-                B[j] = qmul(B[j], link_at(P, 0, (((uint32_t)pR ^ odd) * g.vh + cbR[j]) * LB));
+        for (int k = 0; k < 8; k++) {
+            const int Tt = Tb + k;
+            acc[k] = 0.0;
+            if (Tt > grp.Tmax) continue;
+            const int slot = slots[grp.slot_off + Tt - 1];
+            const uint32_t odd = (uint32_t)(t0 & 1);
+#pragma unroll
+            for (int j = 0; j < NS; j++) {
+                if (Tt > 1) {
+                    Lx[j] = qmul(Lx[j], link_at(P, 0, (((uint32_t)p ^ odd) * g.vh + cbT[j]) * LB));
+                    B[j] = qmul(B[j], link_at(P, 0, (((uint32_t)pR ^ odd) * g.vh + cbR[j]) * LB));
+                }
+                // one step in t
+                cbT[j] += (tT[j] + 1 == g.L[0]) ? (uint32_t)0 - wrap0 : st0;
+                tT[j] = (tT[j] + 1 == g.L[0]) ? 0 : tT[j] + 1;
+                cbR[j] += (tR[j] + 1 == g.L[0]) ? (uint32_t)0 - wrap0 : st0;
+                tR[j] = (tR[j] + 1 == g.L[0]) ? 0 : tR[j] + 1;
+                if (slot >= 0) {
+                    const Q<T> St = link_at(S, d, (((uint32_t)p ^ odd ^ 1u) * g.vh + cbT[j]) * LB);
+                    const T w = T(2) * qdot(qmul(Lx[j], St), B[j]);
+                    acc[k] += valid[j] ? (double)w : 0.0;
+                }
             }
-            // one step in t
-            cbT[j] += (tT[j] + 1 == g.L[0]) ? (uint32_t)0 - wrap0 : st0;
-            tT[j] = (tT[j] + 1 == g.L[0]) ? 0 : tT[j] + 1;
-            cbR[j] += (tR[j] + 1 == g.L[0]) ? (uint32_t)0 - wrap0 : st0;
-            tR[j] = (tR[j] + 1 == g.L[0]) ? 0 : tR[j] + 1;
-            if (slot >= 0) {
-                const Q<T> St = link_at(S, d, (((uint32_t)p ^ odd ^ 1u) * g.vh + cbT[j]) * LB);
-                const T w = T(2) * qdot(qmul(Lx[j], St), B[j]);
-                acc += valid[j] ? (double)w : 0.0;
-            }
+            t0 = (t0 + 1 == g.L[0]) ? 0 : t0 + 1;
         }
-        t0 = (t0 + 1 == g.L[0]) ? 0 : t0 + 1;
-        if (slot >= 0) {
-            const DD v = wave_sum_dd(DD{acc, 0.0});
-            if (lane == 0) wacc[wid][Tt - 1] = v;
-        }
+        const DD v = wave_sum8_dd(acc, lane);
+        const int Tt = Tb + (lane >> 3);
+        if ((lane & 7) == 0 && Tt <= grp.Tmax) wacc[wid][Tt - 1] = v;
     }
     __syncthreads();
     for (int t = threadIdx.x; t < grp.Tmax; t += NT) {
