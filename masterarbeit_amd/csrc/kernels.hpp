@@ -1081,7 +1081,10 @@ __device__ __forceinline__ DD wave_sum8_dd(const double (&v)[8], int lane)
     return x;
 }
 
-// One group (one R, all its T) per launch; S holds the lines S_R.  A block
+// One group (one R, all its T) per launch; S holds the lines S_R.  With
+// Rn > 0 the launch also writes the next group's lines S_Rn of its own sites
+// into Sn (a second line field: S is still read by other blocks), so the line
+// field is built by k_sline only before the first group.  A block
 // covers one 256-site tile with WL<T>::THREADS lanes, each taking WL<T>::SITES
 // sites of it (stride THREADS): the lane sums its sites' values per T in
 // double, and the wave sums of eight T at a time are taken together
@@ -1099,7 +1102,7 @@ template <typename T>
 __global__ void __launch_bounds__(WL<T>::THREADS) k_wloops(Planes<T> P, Planes<T> S, Geo g,
                                                           const WGroup* __restrict__ groups, int group,
                                                           const int* __restrict__ slots, DD* __restrict__ partial,
-                                                          int ncols)
+                                                          int ncols, Planes<T> Sn, int Rn)
 {
     constexpr int NS = WL<T>::SITES, NT = WL<T>::THREADS, NW = WL<T>::NW;
     __shared__ DD wacc[NW][WMAXT];
@@ -1130,7 +1133,21 @@ __global__ void __launch_bounds__(WL<T>::THREADS) k_wloops(Planes<T> P, Planes<T
         tR[j] = sR.c[0];
         cbR[j] = cb_index(g, sR);
         Lx[j] = link(P, g, 0, s0);
-        B[j] = qmul(link(S, g, d, s0), link(P, g, 0, sR));
+        const Q<T> S0 = link(S, g, d, s0);
+        B[j] = qmul(S0, link(P, g, 0, sR));
+        if (Rn > 0 && valid[j]) {
+            // the next group's line S_Rn(x) = S_R(x) U_d(x+Rd) ... U_d(x+(Rn-1)d),
+            // the products in k_sline's order
+            Q<T> line = S0;
+            Site y = sR;
+            for (int r = R; r < Rn; r++) {
+                line = qmul(line, link(P, g, d, y));
+                y = shift(g, y, d, 1);
+            }
+            typename Store<T>::V v;
+            v.x = line.c0; v.y = line.c1; v.z = line.c2; v.w = line.c3;
+            Sn.p[d][p][cbT[j]] = v;   // read by the next launch: a cached store
+        }
     }
     int t0 = 0;
     // eight T per batch; the lanes' sums of a batch are reduced together

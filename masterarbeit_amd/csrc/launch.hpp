@@ -41,8 +41,10 @@ template <typename T> struct Launch {
     static hipError_t ape(hipStream_t s, const Geo& g, const RawPlanes& in, const RawPlanes& out, int mode, double sp);
     // groups/slots: device tables; partial: nloop rows of 6*ceil(vh/256) columns
     static hipError_t sline(hipStream_t s, const Geo& g, const RawPlanes& P, const RawPlanes& S, int R0, int R1);
+    // Rn > 0: also writes the next group's lines S_Rn into Sn
     static hipError_t wloops(hipStream_t s, const Geo& g, const RawPlanes& P, const RawPlanes& S,
-                             const WGroupHost* groups, int group, const int* slots, void* partial, int ncols);
+                             const WGroupHost* groups, int group, const int* slots, void* partial, int ncols,
+                             const RawPlanes& Sn, int Rn);
     // partial: 2 rows of 2*ceil(vh/256) columns (sum q, sum |q|); the improved
     // charge (kind 0) reads the line fields L2, L3 that qlines builds from P
     static hipError_t qlines(hipStream_t s, const Geo& g, const RawPlanes& P, const RawPlanes& L2,

@@ -176,11 +176,12 @@ hipError_t Launch<T>::sline(hipStream_t s, const Geo& g, const RawPlanes& P, con
 
 template <typename T>
 hipError_t Launch<T>::wloops(hipStream_t s, const Geo& g, const RawPlanes& P, const RawPlanes& S,
-                             const WGroupHost* groups, int group, const int* slots, void* partial, int ncols)
+                             const WGroupHost* groups, int group, const int* slots, void* partial, int ncols,
+                             const RawPlanes& Sn, int Rn)
 {
     dim3 grid(2 * grid_x(g), 3), block(WL<T>::THREADS);
     hipLaunchKernelGGL(k_wloops<T>, grid, block, 0, s, planes<T>(P), planes<T>(S), g, (const WGroup*)groups, group,
-                       slots, (DD*)partial, ncols);
+                       slots, (DD*)partial, ncols, planes<T>(Sn), Rn);
     return hipGetLastError();
 }
 

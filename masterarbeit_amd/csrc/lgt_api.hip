@@ -103,6 +103,8 @@ struct lgt_engine {
     hipStream_t stream = nullptr;
     void* field = nullptr;
     void* field_alt = nullptr;
+    void* wline = nullptr;      // Wilson loops: the second spatial-line field (directions 1..3)
+    RawPlanes wl2{};
     RawPlanes cur, alt;
     void* partial = nullptr;
     size_t partial_bytes = 0;
@@ -592,6 +594,7 @@ int lgt_destroy(lgt_engine* e)
     free_vjp(e, false);
     if (e->field) hipFree(e->field);
     if (e->field_alt) hipFree(e->field_alt);
+    if (e->wline) hipFree(e->wline);
     if (e->partial) hipFree(e->partial);
     if (e->dres) hipFree(e->dres);
     if (e->staging) hipFree(e->staging);
@@ -1067,20 +1070,31 @@ int lgt_ape(lgt_engine* e, int mode, int nsteps, double sp)
 
 // the loops of `groups` (R ascending) into the partial rows: for each group
 // the line field (alt planes 1..3) is advanced to its R, then its loops run
+// The spatial lines S_R of the groups (R ascending) alternate between the APE
+// buffer and a second line field: k_sline builds those of the first group,
+// every loop launch but the last writes the next group's.
 static int run_wloops(lgt_engine* e, const std::vector<WGroupHost>& groups, const WGroupHost* dgroups,
                       const int* dslots, int ncols)
 {
-    int Rcur = 0;
+    if (groups.size() > 1 && !e->wline) {
+        size_t have = 0;
+        TRY(ensure(e, &e->wline, &have, (size_t)3 * e->vol * e->link_bytes, "second line field"));
+        for (int d = 0; d < 4; d++)
+            for (int p = 0; p < 2; p++)
+                e->wl2.p[d][p] = d == 0 ? nullptr
+                                        : (char*)e->wline + ((size_t)((d - 1) * 2 + p) * e->geo.vh) * e->link_bytes;
+    }
+    RawPlanes S = e->alt, Sn = e->wl2;
+    TRY(run(e, LGT_K_WLOOP, "spatial lines", [&] {
+        return DISPATCH(e, sline(e->stream, kgeo(e, LGT_K_WLOOP, 0), e->cur, S, 0, groups[0].R));
+    }));
     for (size_t k = 0; k < groups.size(); k++) {
-        const int R = groups[k].R;
-        TRY(run(e, LGT_K_WLOOP, "spatial lines", [&] {
-            return DISPATCH(e, sline(e->stream, kgeo(e, LGT_K_WLOOP, 0), e->cur, e->alt, Rcur, R));
-        }));
-        Rcur = R;
+        const int Rn = (k + 1 < groups.size()) ? groups[k + 1].R : 0;
         TRY(run(e, LGT_K_WLOOP, "wilson loops", [&] {
-            return DISPATCH(e, wloops(e->stream, kgeo(e, LGT_K_WLOOP, 0), e->cur, e->alt, dgroups, (int)k, dslots,
-                                      e->partial, ncols));
+            return DISPATCH(e, wloops(e->stream, kgeo(e, LGT_K_WLOOP, 0), e->cur, S, dgroups, (int)k, dslots,
+                                      e->partial, ncols, Sn, Rn));
         }));
+        std::swap(S, Sn);
     }
     return LGT_OK;
 }
