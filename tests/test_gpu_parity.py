@@ -194,6 +194,26 @@ def test_wilson_loop_unsorted_repeated_and_wrapping(lgt, prec):
     assert w[0] == w[3]
 
 
+@pytest.mark.parametrize("prec", [0, 1])
+def test_wilson_loop_long_t(lgt, prec):
+    """T beyond 8: the loop kernel sums eight T at a time across a wave, so
+    this covers a full batch, a partial one (T up to 11), T values missing
+    from a group, and spatial lines advanced by several links between groups
+    (R = 1, 2, 5, 6)."""
+    L = (12, 4, 4, 6)
+    o, U = thermal_field(L)
+    o.ape(U, 0, 4, 0.25)
+    R = np.array([1] * 11 + [2, 2, 2, 5, 5, 6])
+    T = np.array(list(range(1, 12)) + [2, 9, 11, 10, 3, 8])
+    ref = o.wilson(U, R, T)
+    with engine(lgt, L, prec, U=U) as e:
+        w = e.wilson_loops(R, T)
+        single = np.array([e.wilson_loops([r], [t])[0] for r, t in zip(R, T)])
+    tol = dict(rtol=1e-12, atol=1e-15) if prec == 0 else dict(rtol=0, atol=2e-6)
+    np.testing.assert_allclose(w, ref, **tol)
+    np.testing.assert_array_equal(single, w)
+
+
 def test_wilson_loop_reference_fixture(lgt):
     for name in ["ref_l4.npz", "ref_a6448.npz"]:
         fx = load(name)
