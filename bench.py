@@ -283,9 +283,10 @@ def time_observables(e, L):
     vol = int(np.prod(L))
     R, T = np.meshgrid(np.arange(1, 11), np.arange(1, 17), indexing="ij")
     R, T = R.ravel(), T.ravel()
-    # untimed first calls: they allocate the loops' line field and the
-    # charge's two line fields, which later calls reuse
-    e.wilson_loops(R[:1], T[:1])
+    # untimed first calls: they allocate the loops' two line fields (a second
+    # one only when the loops span more than one R) and the charge's two line
+    # fields, which later calls reuse
+    e.wilson_loops([1, 2], [1, 1])
     e.qtop(0)
     e.sync()
     t0 = time.perf_counter()
