@@ -51,8 +51,8 @@ __global__ void __launch_bounds__(BLOCK) k_vjp_tangent(Planes<T> U, Planes<T> G,
     Q<T> gr = qload<T>(G.p[mu][p], i);
     if (D::smeared(mu)) {
         Q<T> u = qload<T>(U.p[mu][p], i);
-        T c = qdot(gr, u), m = Mn.p[mu][p][i];
-        gr = Q<T>{(gr.c0 - c * u.c0) / m, (gr.c1 - c * u.c1) / m, (gr.c2 - c * u.c2) / m, (gr.c3 - c * u.c3) / m};
+        const T c = qdot(gr, u), r = T(1) / Mn.p[mu][p][i];
+        gr = Q<T>{(gr.c0 - c * u.c0) * r, (gr.c1 - c * u.c1) * r, (gr.c2 - c * u.c2) * r, (gr.c3 - c * u.c3) * r};
     }
     qstore<T>(H.p[mu][p], i, gr);
 }
@@ -165,8 +165,8 @@ __global__ void __launch_bounds__(BLOCK) k_vjp_step(Planes<T> Z, Planes<T> Hin, 
         Q<T> G = qadd(Hy[a], qscale(acc[a], sp));
         if (has_prev && D::smeared(a)) {
             const Q<T>& z = Zy[a];
-            T c = qdot(G, z), m = Mprev.p[a][p][i];
-            G = Q<T>{(G.c0 - c * z.c0) / m, (G.c1 - c * z.c1) / m, (G.c2 - c * z.c2) / m, (G.c3 - c * z.c3) / m};
+            const T c = qdot(G, z), r = T(1) / Mprev.p[a][p][i];
+            G = Q<T>{(G.c0 - c * z.c0) * r, (G.c1 - c * z.c1) * r, (G.c2 - c * z.c2) * r, (G.c3 - c * z.c3) * r};
         }
         qstore<T>(Out.p[a][p], i, G);
     }
