@@ -28,6 +28,56 @@
 namespace lgt {
 
 
+// acc + v w and acc + v w^+ in FP32 storage: the sum rides in each
+// component's FMA chain (four FMAs per component instead of a product and an
+// addition).  FP64 keeps the separate products and sums, in the same order
+// as before.
+template <typename T>
+__device__ __forceinline__ Q<T> mul_acc(const Q<T>& a, const Q<T>& v, const Q<T>& w)
+{
+    if constexpr (sizeof(T) == 4) {
+        Q<T> o;
+        o.c0 = a.c0 + v.c0 * w.c0 - v.c1 * w.c1 - v.c2 * w.c2 - v.c3 * w.c3;
+        o.c1 = a.c1 + v.c0 * w.c1 + v.c1 * w.c0 - v.c2 * w.c3 + v.c3 * w.c2;
+        o.c2 = a.c2 + v.c0 * w.c2 + v.c1 * w.c3 + v.c2 * w.c0 - v.c3 * w.c1;
+        o.c3 = a.c3 + v.c0 * w.c3 - v.c1 * w.c2 + v.c2 * w.c1 + v.c3 * w.c0;
+        return o;
+    } else {
+        return qadd(a, qmul(v, w));
+    }
+}
+
+template <typename T>
+__device__ __forceinline__ Q<T> mul_dag_acc(const Q<T>& a, const Q<T>& v, const Q<T>& w)
+{
+    if constexpr (sizeof(T) == 4) {
+        Q<T> o;
+        o.c0 = a.c0 + v.c0 * w.c0 + v.c1 * w.c1 + v.c2 * w.c2 + v.c3 * w.c3;
+        o.c1 = a.c1 - v.c0 * w.c1 + v.c1 * w.c0 + v.c2 * w.c3 - v.c3 * w.c2;
+        o.c2 = a.c2 - v.c0 * w.c2 - v.c1 * w.c3 + v.c2 * w.c0 + v.c3 * w.c1;
+        o.c3 = a.c3 - v.c0 * w.c3 + v.c1 * w.c2 - v.c2 * w.c1 + v.c3 * w.c0;
+        return o;
+    } else {
+        return qadd(a, qmul_dag(v, w));
+    }
+}
+
+// acc + (u w^+ + v x^+) and acc + (u w + v x): FP32 folds both products into
+// the sum; FP64 forms the pair first, then adds it, as before
+template <typename T>
+__device__ __forceinline__ Q<T> add_terms(const Q<T>& acc, const Q<T>& u, const Q<T>& w, const Q<T>& v, const Q<T>& x)
+{
+    if constexpr (sizeof(T) == 4) return mul_dag_acc(mul_dag_acc(acc, u, w), v, x);
+    else return qadd(acc, qadd(qmul_dag(u, w), qmul_dag(v, x)));
+}
+
+template <typename T>
+__device__ __forceinline__ Q<T> add_terms_fwd(const Q<T>& acc, const Q<T>& u, const Q<T>& w, const Q<T>& v, const Q<T>& x)
+{
+    if constexpr (sizeof(T) == 4) return mul_acc(mul_acc(acc, u, w), v, x);
+    else return qadd(acc, qadd(qmul(u, w), qmul(v, x)));
+}
+
 // Directions of the APE modes (LGT_Code/modules/obs/smearing.c:74-168): the
 // smeared links j in [LO, HI]; the staple directions k of j; the directions
 // whose cotangent the adjoint carries, [CLO, 3] (all that a staple touches).
@@ -107,19 +157,15 @@ __global__ void __launch_bounds__(BLOCK) k_vjp_step(Planes<T> Z, Planes<T> Hin, 
             asm volatile("" ::: "memory");
             // t1 + t5 + t4 of G_a = (H_b(y) Z_a(y+b) + Z_b(y) H_a(y+b)) Z_b(y+a)^+ + Z_b(y) Z_a(y+b) H_b(y+a)^+
             if (side_ab || mid_ab) {
-                const Q<T> up = (side_ab && mid_ab) ? qadd(qmul(Hy[b], Zab), qmul(Zy[b], Hab))
+                const Q<T> up = (side_ab && mid_ab) ? mul_acc(qmul(Hy[b], Zab), Zy[b], Hab)
                                                    : (side_ab ? qmul(Hy[b], Zab) : qmul(Zy[b], Hab));
-                Q<T> t = qmul_dag(up, Zba);
-                if (side_ab) t = qadd(t, qmul_dag(qmul(Zy[b], Zab), Hba));
-                acc[a] = qadd(acc[a], t);
+                acc[a] = side_ab ? add_terms(acc[a], up, Zba, qmul(Zy[b], Zab), Hba) : mul_dag_acc(acc[a], up, Zba);
             }
             // the same for G_b, a and b exchanged
             if (side_ba || mid_ba) {
-                const Q<T> up = (side_ba && mid_ba) ? qadd(qmul(Hy[a], Zba), qmul(Zy[a], Hba))
+                const Q<T> up = (side_ba && mid_ba) ? mul_acc(qmul(Hy[a], Zba), Zy[a], Hba)
                                                    : (side_ba ? qmul(Hy[a], Zba) : qmul(Zy[a], Hba));
-                Q<T> t = qmul_dag(up, Zab);
-                if (side_ba) t = qadd(t, qmul_dag(qmul(Zy[a], Zba), Hab));
-                acc[b] = qadd(acc[b], t);
+                acc[b] = side_ba ? add_terms(acc[b], up, Zab, qmul(Zy[a], Zba), Hab) : mul_dag_acc(acc[b], up, Zab);
             }
         }
     // backward staples: for each b the links around y-b
@@ -153,11 +199,9 @@ __global__ void __launch_bounds__(BLOCK) k_vjp_step(Planes<T> Z, Planes<T> Hin, 
             const bool side = D::staple(b, a), mid = D::staple(a, b);
             if (a == b || !(side || mid)) continue;
             // t2 + t3 + t6 = (Z_b(y-b)^+ H_a(y-b) + H_b(y-b)^+ Z_a(y-b)) Z_b(y-b+a) + Z_b(y-b)^+ Z_a(y-b) H_b(y-b+a)
-            const Q<T> lo = (side && mid) ? qadd(qmul(Zbd, C[a][1]), qmul(Hbd, C[a][0]))
+            const Q<T> lo = (side && mid) ? mul_acc(qmul(Zbd, C[a][1]), Hbd, C[a][0])
                                           : (mid ? qmul(Zbd, C[a][1]) : qmul(Hbd, C[a][0]));
-            Q<T> t = qmul(lo, E[a][0]);
-            if (side) t = qadd(t, qmul(qmul(Zbd, C[a][0]), E[a][1]));
-            acc[a] = qadd(acc[a], t);
+            acc[a] = side ? add_terms_fwd(acc[a], lo, E[a][0], qmul(Zbd, C[a][0]), E[a][1]) : mul_acc(acc[a], lo, E[a][0]);
         }
     }
 #pragma unroll
