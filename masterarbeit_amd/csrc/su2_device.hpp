@@ -80,6 +80,30 @@ __device__ __forceinline__ Q<T> qmul_dag(const Q<T>& v, const Q<T>& w)
     return o;
 }
 
+// a + v*w and a + v*w^+: each component one chain, so that FMA contraction
+// folds the sum into the product (the FP32 adjoint)
+template <typename T>
+__device__ __forceinline__ Q<T> qmul_add(const Q<T>& a, const Q<T>& v, const Q<T>& w)
+{
+    Q<T> o;
+    o.c0 = a.c0 + v.c0 * w.c0 - v.c1 * w.c1 - v.c2 * w.c2 - v.c3 * w.c3;
+    o.c1 = a.c1 + v.c0 * w.c1 + v.c1 * w.c0 - v.c2 * w.c3 + v.c3 * w.c2;
+    o.c2 = a.c2 + v.c0 * w.c2 + v.c1 * w.c3 + v.c2 * w.c0 - v.c3 * w.c1;
+    o.c3 = a.c3 + v.c0 * w.c3 - v.c1 * w.c2 + v.c2 * w.c1 + v.c3 * w.c0;
+    return o;
+}
+
+template <typename T>
+__device__ __forceinline__ Q<T> qmul_dag_add(const Q<T>& a, const Q<T>& v, const Q<T>& w)
+{
+    Q<T> o;
+    o.c0 = a.c0 + v.c0 * w.c0 + v.c1 * w.c1 + v.c2 * w.c2 + v.c3 * w.c3;
+    o.c1 = a.c1 - v.c0 * w.c1 + v.c1 * w.c0 + v.c2 * w.c3 - v.c3 * w.c2;
+    o.c2 = a.c2 - v.c0 * w.c2 - v.c1 * w.c3 + v.c2 * w.c0 + v.c3 * w.c1;
+    o.c3 = a.c3 - v.c0 * w.c3 + v.c1 * w.c2 - v.c2 * w.c1 + v.c3 * w.c0;
+    return o;
+}
+
 // v^+ * w, written as the reference writes it: dagger first, then product
 template <typename T>
 __device__ __forceinline__ Q<T> qdag(const Q<T>& v) { return Q<T>{v.c0, -v.c1, -v.c2, -v.c3}; }

@@ -28,38 +28,20 @@
 namespace lgt {
 
 
-// acc + v w and acc + v w^+ in FP32 storage: the sum rides in each
-// component's FMA chain (four FMAs per component instead of a product and an
-// addition).  FP64 keeps the separate products and sums, in the same order
-// as before.
+// acc + v w and acc + v w^+: in FP32 storage the sum rides in each
+// component's FMA chain (qmul_add); FP64 keeps the separate products and sums
 template <typename T>
 __device__ __forceinline__ Q<T> mul_acc(const Q<T>& a, const Q<T>& v, const Q<T>& w)
 {
-    if constexpr (sizeof(T) == 4) {
-        Q<T> o;
-        o.c0 = a.c0 + v.c0 * w.c0 - v.c1 * w.c1 - v.c2 * w.c2 - v.c3 * w.c3;
-        o.c1 = a.c1 + v.c0 * w.c1 + v.c1 * w.c0 - v.c2 * w.c3 + v.c3 * w.c2;
-        o.c2 = a.c2 + v.c0 * w.c2 + v.c1 * w.c3 + v.c2 * w.c0 - v.c3 * w.c1;
-        o.c3 = a.c3 + v.c0 * w.c3 - v.c1 * w.c2 + v.c2 * w.c1 + v.c3 * w.c0;
-        return o;
-    } else {
-        return qadd(a, qmul(v, w));
-    }
+    if constexpr (sizeof(T) == 4) return qmul_add(a, v, w);
+    else return qadd(a, qmul(v, w));
 }
 
 template <typename T>
 __device__ __forceinline__ Q<T> mul_dag_acc(const Q<T>& a, const Q<T>& v, const Q<T>& w)
 {
-    if constexpr (sizeof(T) == 4) {
-        Q<T> o;
-        o.c0 = a.c0 + v.c0 * w.c0 + v.c1 * w.c1 + v.c2 * w.c2 + v.c3 * w.c3;
-        o.c1 = a.c1 - v.c0 * w.c1 + v.c1 * w.c0 + v.c2 * w.c3 - v.c3 * w.c2;
-        o.c2 = a.c2 - v.c0 * w.c2 - v.c1 * w.c3 + v.c2 * w.c0 + v.c3 * w.c1;
-        o.c3 = a.c3 - v.c0 * w.c3 + v.c1 * w.c2 - v.c2 * w.c1 + v.c3 * w.c0;
-        return o;
-    } else {
-        return qadd(a, qmul_dag(v, w));
-    }
+    if constexpr (sizeof(T) == 4) return qmul_dag_add(a, v, w);
+    else return qadd(a, qmul_dag(v, w));
 }
 
 // acc + (u w^+ + v x^+) and acc + (u w + v x): FP32 folds both products into
