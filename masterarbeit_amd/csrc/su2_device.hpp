@@ -80,11 +80,37 @@ __device__ __forceinline__ Q<T> qmul_dag(const Q<T>& v, const Q<T>& w)
     return o;
 }
 
-// a + v*w and a + v*w^+: each component one chain, so that FMA contraction
-// folds the sum into the product (the FP32 adjoint)
+// Products for code with no reference rounding to reproduce (the adjoint of
+// the smearing): FMA contraction in either precision.  v*w, v*w^+, and
+// a + v*w, a + v*w^+ with the sum in each component's FMA chain.
+template <typename T>
+__device__ __forceinline__ Q<T> qmul_c(const Q<T>& v, const Q<T>& w)
+{
+#pragma clang fp contract(fast)
+    Q<T> o;
+    o.c0 = v.c0 * w.c0 - v.c1 * w.c1 - v.c2 * w.c2 - v.c3 * w.c3;
+    o.c1 = v.c0 * w.c1 + v.c1 * w.c0 - v.c2 * w.c3 + v.c3 * w.c2;
+    o.c2 = v.c0 * w.c2 + v.c1 * w.c3 + v.c2 * w.c0 - v.c3 * w.c1;
+    o.c3 = v.c0 * w.c3 - v.c1 * w.c2 + v.c2 * w.c1 + v.c3 * w.c0;
+    return o;
+}
+
+template <typename T>
+__device__ __forceinline__ Q<T> qmul_dag_c(const Q<T>& v, const Q<T>& w)
+{
+#pragma clang fp contract(fast)
+    Q<T> o;
+    o.c0 = v.c0 * w.c0 + v.c1 * w.c1 + v.c2 * w.c2 + v.c3 * w.c3;
+    o.c1 = -v.c0 * w.c1 + v.c1 * w.c0 + v.c2 * w.c3 - v.c3 * w.c2;
+    o.c2 = -v.c0 * w.c2 - v.c1 * w.c3 + v.c2 * w.c0 + v.c3 * w.c1;
+    o.c3 = -v.c0 * w.c3 + v.c1 * w.c2 - v.c2 * w.c1 + v.c3 * w.c0;
+    return o;
+}
+
 template <typename T>
 __device__ __forceinline__ Q<T> qmul_add(const Q<T>& a, const Q<T>& v, const Q<T>& w)
 {
+#pragma clang fp contract(fast)
     Q<T> o;
     o.c0 = a.c0 + v.c0 * w.c0 - v.c1 * w.c1 - v.c2 * w.c2 - v.c3 * w.c3;
     o.c1 = a.c1 + v.c0 * w.c1 + v.c1 * w.c0 - v.c2 * w.c3 + v.c3 * w.c2;
@@ -96,6 +122,7 @@ __device__ __forceinline__ Q<T> qmul_add(const Q<T>& a, const Q<T>& v, const Q<T
 template <typename T>
 __device__ __forceinline__ Q<T> qmul_dag_add(const Q<T>& a, const Q<T>& v, const Q<T>& w)
 {
+#pragma clang fp contract(fast)
     Q<T> o;
     o.c0 = a.c0 + v.c0 * w.c0 + v.c1 * w.c1 + v.c2 * w.c2 + v.c3 * w.c3;
     o.c1 = a.c1 - v.c0 * w.c1 + v.c1 * w.c0 + v.c2 * w.c3 - v.c3 * w.c2;

@@ -28,36 +28,18 @@
 namespace lgt {
 
 
-// acc + v w and acc + v w^+: in FP32 storage the sum rides in each
-// component's FMA chain (qmul_add); FP64 keeps the separate products and sums
-template <typename T>
-__device__ __forceinline__ Q<T> mul_acc(const Q<T>& a, const Q<T>& v, const Q<T>& w)
-{
-    if constexpr (sizeof(T) == 4) return qmul_add(a, v, w);
-    else return qadd(a, qmul(v, w));
-}
-
-template <typename T>
-__device__ __forceinline__ Q<T> mul_dag_acc(const Q<T>& a, const Q<T>& v, const Q<T>& w)
-{
-    if constexpr (sizeof(T) == 4) return qmul_dag_add(a, v, w);
-    else return qadd(a, qmul_dag(v, w));
-}
-
-// acc + (u w^+ + v x^+) and acc + (u w + v x): FP32 folds both products into
-// the sum; FP64 forms the pair first, then adds it, as before
+// acc + (u w^+ + v x^+) and acc + (u w + v x), both products folded into the
+// sum's FMA chains
 template <typename T>
 __device__ __forceinline__ Q<T> add_terms(const Q<T>& acc, const Q<T>& u, const Q<T>& w, const Q<T>& v, const Q<T>& x)
 {
-    if constexpr (sizeof(T) == 4) return mul_dag_acc(mul_dag_acc(acc, u, w), v, x);
-    else return qadd(acc, qadd(qmul_dag(u, w), qmul_dag(v, x)));
+    return qmul_dag_add(qmul_dag_add(acc, u, w), v, x);
 }
 
 template <typename T>
 __device__ __forceinline__ Q<T> add_terms_fwd(const Q<T>& acc, const Q<T>& u, const Q<T>& w, const Q<T>& v, const Q<T>& x)
 {
-    if constexpr (sizeof(T) == 4) return mul_acc(mul_acc(acc, u, w), v, x);
-    else return qadd(acc, qadd(qmul(u, w), qmul(v, x)));
+    return qmul_add(qmul_add(acc, u, w), v, x);
 }
 
 // Directions of the APE modes (LGT_Code/modules/obs/smearing.c:74-168): the
@@ -139,15 +121,15 @@ __global__ void __launch_bounds__(BLOCK) k_vjp_step(Planes<T> Z, Planes<T> Hin, 
             asm volatile("" ::: "memory");
             // t1 + t5 + t4 of G_a = (H_b(y) Z_a(y+b) + Z_b(y) H_a(y+b)) Z_b(y+a)^+ + Z_b(y) Z_a(y+b) H_b(y+a)^+
             if (side_ab || mid_ab) {
-                const Q<T> up = (side_ab && mid_ab) ? mul_acc(qmul(Hy[b], Zab), Zy[b], Hab)
-                                                   : (side_ab ? qmul(Hy[b], Zab) : qmul(Zy[b], Hab));
-                acc[a] = side_ab ? add_terms(acc[a], up, Zba, qmul(Zy[b], Zab), Hba) : mul_dag_acc(acc[a], up, Zba);
+                const Q<T> up = (side_ab && mid_ab) ? qmul_add(qmul_c(Hy[b], Zab), Zy[b], Hab)
+                                                   : (side_ab ? qmul_c(Hy[b], Zab) : qmul_c(Zy[b], Hab));
+                acc[a] = side_ab ? add_terms(acc[a], up, Zba, qmul_c(Zy[b], Zab), Hba) : qmul_dag_add(acc[a], up, Zba);
             }
             // the same for G_b, a and b exchanged
             if (side_ba || mid_ba) {
-                const Q<T> up = (side_ba && mid_ba) ? mul_acc(qmul(Hy[a], Zba), Zy[a], Hba)
-                                                   : (side_ba ? qmul(Hy[a], Zba) : qmul(Zy[a], Hba));
-                acc[b] = side_ba ? add_terms(acc[b], up, Zab, qmul(Zy[a], Zba), Hab) : mul_dag_acc(acc[b], up, Zab);
+                const Q<T> up = (side_ba && mid_ba) ? qmul_add(qmul_c(Hy[a], Zba), Zy[a], Hba)
+                                                   : (side_ba ? qmul_c(Hy[a], Zba) : qmul_c(Zy[a], Hba));
+                acc[b] = side_ba ? add_terms(acc[b], up, Zab, qmul_c(Zy[a], Zba), Hab) : qmul_dag_add(acc[b], up, Zab);
             }
         }
     // backward staples: for each b the links around y-b
@@ -181,9 +163,9 @@ __global__ void __launch_bounds__(BLOCK) k_vjp_step(Planes<T> Z, Planes<T> Hin, 
             const bool side = D::staple(b, a), mid = D::staple(a, b);
             if (a == b || !(side || mid)) continue;
             // t2 + t3 + t6 = (Z_b(y-b)^+ H_a(y-b) + H_b(y-b)^+ Z_a(y-b)) Z_b(y-b+a) + Z_b(y-b)^+ Z_a(y-b) H_b(y-b+a)
-            const Q<T> lo = (side && mid) ? mul_acc(qmul(Zbd, C[a][1]), Hbd, C[a][0])
-                                          : (mid ? qmul(Zbd, C[a][1]) : qmul(Hbd, C[a][0]));
-            acc[a] = side ? add_terms_fwd(acc[a], lo, E[a][0], qmul(Zbd, C[a][0]), E[a][1]) : mul_acc(acc[a], lo, E[a][0]);
+            const Q<T> lo = (side && mid) ? qmul_add(qmul_c(Zbd, C[a][1]), Hbd, C[a][0])
+                                          : (mid ? qmul_c(Zbd, C[a][1]) : qmul_c(Hbd, C[a][0]));
+            acc[a] = side ? add_terms_fwd(acc[a], lo, E[a][0], qmul_c(Zbd, C[a][0]), E[a][1]) : qmul_add(acc[a], lo, E[a][0]);
         }
     }
 #pragma unroll
