@@ -1163,8 +1163,8 @@ __global__ void __launch_bounds__(WL<T>::THREADS) k_wloops(Planes<T> P, Planes<T
 #pragma unroll
             for (int j = 0; j < NS; j++) {
                 if (Tt > 1) {
-                    Lx[j] = qmul(Lx[j], link_at(P, 0, (((uint32_t)p ^ odd) * g.vh + cbT[j]) * LB));
-                    B[j] = qmul(B[j], link_at(P, 0, (((uint32_t)pR ^ odd) * g.vh + cbR[j]) * LB));
+                    Lx[j] = qmul_c(Lx[j], link_at(P, 0, (((uint32_t)p ^ odd) * g.vh + cbT[j]) * LB));
+                    B[j] = qmul_c(B[j], link_at(P, 0, (((uint32_t)pR ^ odd) * g.vh + cbR[j]) * LB));
                 }
                 // one step in t
                 cbT[j] += (tT[j] + 1 == g.L[0]) ? (uint32_t)0 - wrap0 : st0;
@@ -1173,7 +1173,7 @@ __global__ void __launch_bounds__(WL<T>::THREADS) k_wloops(Planes<T> P, Planes<T
                 tR[j] = (tR[j] + 1 == g.L[0]) ? 0 : tR[j] + 1;
                 if (slot >= 0) {
                     const Q<T> St = link_at(S, d, (((uint32_t)p ^ odd ^ 1u) * g.vh + cbT[j]) * LB);
-                    const T w = T(2) * qdot(qmul(Lx[j], St), B[j]);
+                    const T w = T(2) * qdot_c(qmul_c(Lx[j], St), B[j]);
                     acc[k] += valid[j] ? (double)w : 0.0;
                 }
             }

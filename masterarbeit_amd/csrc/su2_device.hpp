@@ -81,8 +81,9 @@ __device__ __forceinline__ Q<T> qmul_dag(const Q<T>& v, const Q<T>& w)
 }
 
 // Products for code with no reference rounding to reproduce (the adjoint of
-// the smearing): FMA contraction in either precision.  v*w, v*w^+, and
-// a + v*w, a + v*w^+ with the sum in each component's FMA chain.
+// the smearing, the Wilson loops' grown lines): FMA contraction in either
+// precision.  v*w, v*w^+, <a,b>, and a + v*w, a + v*w^+ with the sum in each
+// component's FMA chain.
 template <typename T>
 __device__ __forceinline__ Q<T> qmul_c(const Q<T>& v, const Q<T>& w)
 {
@@ -105,6 +106,13 @@ __device__ __forceinline__ Q<T> qmul_dag_c(const Q<T>& v, const Q<T>& w)
     o.c2 = -v.c0 * w.c2 - v.c1 * w.c3 + v.c2 * w.c0 + v.c3 * w.c1;
     o.c3 = -v.c0 * w.c3 + v.c1 * w.c2 - v.c2 * w.c1 + v.c3 * w.c0;
     return o;
+}
+
+template <typename T>
+__device__ __forceinline__ T qdot_c(const Q<T>& a, const Q<T>& b)
+{
+#pragma clang fp contract(fast)
+    return a.c0 * b.c0 + a.c1 * b.c1 + a.c2 * b.c2 + a.c3 * b.c3;
 }
 
 template <typename T>
