@@ -115,14 +115,27 @@ def time_heatbath_only(e, L, beta, steps):
 
 
 def cpu_model():
+    """The host CPU as /proc/cpuinfo names it, with its family, model and
+    clock (some hosts give only a generic model name)."""
+    fields = {}
     try:
         with open("/proc/cpuinfo") as f:
             for line in f:
-                if line.startswith("model name"):
-                    return line.split(":", 1)[1].strip()
+                if ":" not in line:
+                    if fields:
+                        break
+                    continue
+                k, v = (x.strip() for x in line.split(":", 1))
+                fields.setdefault(k, v)
     except OSError:
-        pass
-    return None
+        return None
+    name = fields.get("model name")
+    if name is None:
+        return None
+    extra = [f"{k} {fields[k]}" for k in ("cpu family", "model", "stepping") if k in fields]
+    if "cpu MHz" in fields:
+        extra.append(f"{float(fields['cpu MHz']):.0f} MHz")
+    return f"{name} ({', '.join(extra)})" if extra else name
 
 
 def observables_24(lgt, beta, seed=5, reps=5):
