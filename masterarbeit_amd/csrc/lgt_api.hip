@@ -1068,11 +1068,10 @@ int lgt_ape(lgt_engine* e, int mode, int nsteps, double sp)
     return LGT_OK;
 }
 
-// the loops of `groups` (R ascending) into the partial rows: for each group
-// the line field (alt planes 1..3) is advanced to its R, then its loops run
-// The spatial lines S_R of the groups (R ascending) alternate between the APE
-// buffer and a second line field: k_sline builds those of the first group,
-// every loop launch but the last writes the next group's.
+// the loops of `groups` (R ascending) into the partial rows.  The spatial
+// lines S_R of the groups alternate between the APE buffer (planes 1..3) and a
+// second line field: k_sline builds those of the first group, and every loop
+// launch but the last writes the next group's.
 static int run_wloops(lgt_engine* e, const std::vector<WGroupHost>& groups, const WGroupHost* dgroups,
                       const int* dslots, int ncols)
 {

@@ -82,9 +82,11 @@ __global__ void __launch_bounds__(BLOCK) k_vjp_tangent(Planes<T> U, Planes<T> G,
 // and G_b; for every b those around y-b, which close the backward staples of
 // the G_a, a != b.  A batch's loads are issued together before its products
 // (its latency is exposed once) and only one batch is live at a time
-// (spatial FP64: 160 VGPRs, three waves per SIMD).  A term the mode does not
+// (spatial FP64: 167 VGPRs, three waves per SIMD).  A term the mode does not
 // have is neither loaded nor computed; the spatial mode's arithmetic is the
-// same operation for operation as with every term present.
+// same operation for operation as with every term present.  Products and
+// running sums contract to FMAs in both precisions (qmul_c, qmul_add): the
+// adjoint has no reference rounding to reproduce.
 // grid: x over (tile, parity) pairs
 template <typename T, int MODE>
 __global__ void __launch_bounds__(BLOCK) k_vjp_step(Planes<T> Z, Planes<T> Hin, NormPlanes<T> Mprev, int has_prev,
