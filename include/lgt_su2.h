@@ -220,7 +220,10 @@ int lgt_ape(lgt_engine *e, int mode, int nsteps, double sp);
 /* measureWilsonLoop(w) (LGT_Code/modules/obs/wilson.c:268-324) with the loop
  * list given explicitly (measParams.rExtents/tExtents, wilson.c:274-285):
  * w[i] = mean over sites x and spatial d of (1/2) Re tr W_{R[i] x T[i]}(x; t, d).
- * Requires 1 <= T[i] <= extents[0]-1 (wilson.c:290-291) and R[i] >= 1. */
+ * Requires 1 <= T[i] <= extents[0]-1 (wilson.c:290-291) and R[i] >= 1.
+ * Device memory: the smearing buffer (one field) holds the spatial lines; a
+ * list with more than one R also allocates a second line field (three
+ * directions), kept for later calls. */
 int lgt_wilson_loops(lgt_engine *e, int nloop, const int *R, const int *T, double *w);
 
 /* topologicalCharge() / meas_topologicalcharge(q) (LGT_Code/modules/obs/
