@@ -602,8 +602,9 @@ def main():
             "kernels": r64["kernels"],
         },
         "heatbath_only": {"fp32": hb32, "fp64": hb64,
-                          "note": "one heatbath sweep + one projection pass per step, the reference's algorithm; "
-                                  "compare with cpu_baseline"},
+                          "note": "one heatbath sweep + one projection of the field per step, the reference's "
+                                  "algorithm (FP32: links projected as the heatbath stores them; FP64: a separate "
+                                  "projection pass); compare with cpu_baseline"},
         "observables_24_cpu_vs_gpu": obs24,
         "fp32_qcd_loop_ms_per_step": r32["qcd_loop_ms_per_step"],
         "fp32_over_fp64": r32["link_updates_per_s"] / r64["link_updates_per_s"],

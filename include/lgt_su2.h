@@ -150,7 +150,9 @@ int lgt_read_config_doubled(lgt_engine *e, const char *path, double *plaq_file, 
  * parity 0,1, heatbath.c:76-149 per link) followed by n_or overrelaxation
  * sweeps in the same order (U' = V^+ U^+ V^+, V = A/|A|, links re-projected
  * as they are stored).  With n_or == 0 the whole field is projected after
- * every sweep (update.c:166).  Sweep k draws its uniforms from Philox4x32-10
+ * every sweep (update.c:166); FP32 engines instead store each heatbath link
+ * projected, so later sub-sweeps of the same sweep see projected links
+ * (a difference at FP32 rounding level).  Sweep k draws its uniforms from Philox4x32-10
  * keyed by the seed with counter (cb site, mu|parity<<2, sweep index, draw).
  * acceptance (nullable) = mean over links and sweeps of 1/trials, as the
  * reference logs it (update.c:159,170-173). */

@@ -515,8 +515,9 @@ template <> struct HB<float> {
 };
 
 // heatbath sub-sweep of direction MU, parity p.  acc_partial[tile] receives
-// the block's sum of 1/count (nullable).
-template <typename T, int MU>
+// the block's sum of 1/count (nullable).  PROJ stores the new link projected
+// onto SU(2), which stands in for a separate projection pass after the sweep.
+template <typename T, int MU, bool PROJ>
 __global__ void __launch_bounds__(BLOCK) k_heatbath(Planes<T> P, Geo g, int p, T beta, uint64_t seed, uint32_t sweep,
                                                     DD* __restrict__ acc_partial)
 {
@@ -529,7 +530,7 @@ __global__ void __launch_bounds__(BLOCK) k_heatbath(Planes<T> P, Geo g, int p, T
         Q<T> A = update_staple<T, MU>(P, g, s, p);
         Q<T> Un;
         count = HB<T>::update(A, beta, seed, i, (uint32_t)(MU | (p << 2) | (KIND_HB << 3)), sweep, Un);
-        qstore<T>(P.p[MU][p], i, Un);
+        qstore<T>(P.p[MU][p], i, PROJ ? qproject(Un) : Un);
     }
     if (acc_partial) {
         DD s = block_sum_dd(DD{count > 0 ? 1.0 / count : 0.0, 0.0});

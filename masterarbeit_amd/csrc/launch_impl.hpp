@@ -67,16 +67,21 @@ hipError_t Launch<T>::gauge_elements(hipStream_t s, const Geo& g, uint64_t seed,
 
 template <typename T>
 hipError_t Launch<T>::heatbath(hipStream_t s, const Geo& g, const RawPlanes& P, int mu, int parity, double beta,
-                               uint64_t seed, uint32_t sweep, void* acc_partial)
+                               uint64_t seed, uint32_t sweep, void* acc_partial, bool project)
 {
     dim3 grid(grid_x(g)), block(BLOCK);
     Planes<T> Pl = planes<T>(P);
     DD* ap = (DD*)acc_partial;
-    switch (mu) {
-    case 0: hipLaunchKernelGGL((k_heatbath<T, 0>), grid, block, 0, s, Pl, g, parity, (T)beta, seed, sweep, ap); break;
-    case 1: hipLaunchKernelGGL((k_heatbath<T, 1>), grid, block, 0, s, Pl, g, parity, (T)beta, seed, sweep, ap); break;
-    case 2: hipLaunchKernelGGL((k_heatbath<T, 2>), grid, block, 0, s, Pl, g, parity, (T)beta, seed, sweep, ap); break;
-    default: hipLaunchKernelGGL((k_heatbath<T, 3>), grid, block, 0, s, Pl, g, parity, (T)beta, seed, sweep, ap); break;
+    auto go = [&](auto kernel) { hipLaunchKernelGGL(kernel, grid, block, 0, s, Pl, g, parity, (T)beta, seed, sweep, ap); };
+    switch (mu * 2 + (project ? 1 : 0)) {
+    case 0: go(k_heatbath<T, 0, false>); break;
+    case 1: go(k_heatbath<T, 0, true>); break;
+    case 2: go(k_heatbath<T, 1, false>); break;
+    case 3: go(k_heatbath<T, 1, true>); break;
+    case 4: go(k_heatbath<T, 2, false>); break;
+    case 5: go(k_heatbath<T, 2, true>); break;
+    case 6: go(k_heatbath<T, 3, false>); break;
+    default: go(k_heatbath<T, 3, true>); break;
     }
     return hipGetLastError();
 }

@@ -943,12 +943,19 @@ int lgt_update(lgt_engine* e, int nsweeps, int n_or, double beta, double* accept
         TRY(ensure(e, &e->partial, &e->partial_bytes, sizeof(double) * 2 * 8 * (size_t)nb, "partials"));
         acc_rows.resize(nsweeps);
     }
+    // A heatbath-only sweep ends with a projection of the whole field
+    // (update.c:102-176).  In FP64 it is a pass of its own, so that the staples
+    // of the sweep read the heatbath's unprojected links as the reference's
+    // do; in FP32 the heatbath stores its links projected instead, which
+    // saves that pass (one read and one write of the field).
+    const bool fold_projection = n_or == 0 && e->prec == LGT_FP32;
     for (int s = 0; s < nsweeps; s++) {
         for (int mu = 0; mu < 4; mu++)
             for (int p = 0; p < 2; p++) {
                 void* ap = acceptance ? (char*)e->partial + sizeof(double) * 2 * (size_t)nb * (mu * 2 + p) : nullptr;
                 TRY(run(e, LGT_K_HEATBATH, "heatbath", [&] {
-                    return DISPATCH(e, heatbath(e->stream, ugeo(e, LGT_K_HEATBATH, mu), e->cur, mu, p, beta, e->seed, e->sweep, ap));
+                    return DISPATCH(e, heatbath(e->stream, ugeo(e, LGT_K_HEATBATH, mu), e->cur, mu, p, beta, e->seed,
+                                                e->sweep, ap, fold_projection));
                 }));
             }
         if (acceptance) TRY(reduce_to_host(e, 1, (int)(8 * nb), &acc_rows[s]));
@@ -957,7 +964,8 @@ int lgt_update(lgt_engine* e, int nsweeps, int n_or, double beta, double* accept
                 for (int p = 0; p < 2; p++)
                     TRY(run(e, LGT_K_OVERRELAX, "overrelax",
                             [&] { return DISPATCH(e, overrelax(e->stream, ugeo(e, LGT_K_OVERRELAX, mu), e->cur, mu, p)); }));
-        if (n_or == 0) TRY(run(e, LGT_K_PROJECT, "project", [&] { return DISPATCH(e, project(e->stream, e->geo, e->cur)); }));
+        if (n_or == 0 && !fold_projection)
+            TRY(run(e, LGT_K_PROJECT, "project", [&] { return DISPATCH(e, project(e->stream, e->geo, e->cur)); }));
         e->sweep++;
     }
     if (acceptance) {
