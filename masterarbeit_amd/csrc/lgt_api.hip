@@ -400,6 +400,71 @@ int reduce_to_host(lgt_engine* e, int nrows, int ncols, double* out)
     return LGT_OK;
 }
 
+// Per-sweep acceptance of the update loops: the 8 nb block partials of sweep
+// s are reduced on the stream into the double-double dres[s mod ACC_BATCH],
+// and the host reads a batch of those at once instead of waiting for every
+// sweep.  rows[s] = hi + lo of sweep s, as reduce_to_host gives it.
+constexpr int ACC_BATCH = 1024;
+
+int acc_prepare(lgt_engine* e, int nsweeps, std::vector<double>& rows)
+{
+    const size_t batch = (size_t)std::min(std::max(nsweeps, 1), ACC_BATCH);
+    TRY(ensure(e, &e->partial, &e->partial_bytes, sizeof(double) * 2 * 8 * (size_t)nbx(e), "partials"));
+    TRY(ensure(e, &e->dres, &e->dres_bytes, sizeof(double) * 2 * batch, "results"));
+    TRY(ensure_host(e, sizeof(double) * 2 * batch));
+    rows.assign(nsweeps, 0.0);
+    return LGT_OK;
+}
+
+// enqueues the reduction of sweep s's partials (after its update launches)
+int acc_reduce(lgt_engine* e, int s)
+{
+    return run(e, LGT_K_REDUCE, "reduce", [&] {
+        return launch_reduce_rows(e->stream, e->partial, 1, (int)(8 * nbx(e)), (double*)e->dres + 2 * (s % ACC_BATCH));
+    });
+}
+
+// at the end of sweep s: reads the batch back when it is full or s is the last
+int acc_collect(lgt_engine* e, int s, int nsweeps, std::vector<double>& rows)
+{
+    if ((s + 1) % ACC_BATCH != 0 && s + 1 != nsweeps) return LGT_OK;
+    const int s0 = s - s % ACC_BATCH, n = s + 1 - s0;
+    HIP_TRY(hipMemcpyAsync(e->hres, e->dres, sizeof(double) * 2 * n, hipMemcpyDeviceToHost, e->stream),
+            "hipMemcpyAsync");
+    HIP_TRY(hipStreamSynchronize(e->stream), "hipStreamSynchronize");
+    for (int j = 0; j < n; j++) rows[s0 + j] = e->hres[2 * j] + e->hres[2 * j + 1];
+    return LGT_OK;
+}
+
+// The launches of one update sweep (update.c:102-176): the heatbath
+// sub-sweeps (with acc, each block's sum of 1/trials goes to the partials),
+// n_or overrelaxation sweeps, and after a heatbath-only sweep the projection
+// of the whole field.  In FP64 that projection is a pass of its own, so that
+// the staples of the sweep read the heatbath's unprojected links as the
+// reference's do; in FP32 the heatbath stores its links projected instead,
+// which saves that pass (one read and one write of the field).
+int enqueue_sweep(lgt_engine* e, int n_or, double beta, bool acc)
+{
+    const uint32_t nb = nbx(e);
+    const bool fold_projection = n_or == 0 && e->prec == LGT_FP32;
+    for (int mu = 0; mu < 4; mu++)
+        for (int p = 0; p < 2; p++) {
+            void* ap = acc ? (char*)e->partial + sizeof(double) * 2 * (size_t)nb * (mu * 2 + p) : nullptr;
+            TRY(run(e, LGT_K_HEATBATH, "heatbath", [&] {
+                return DISPATCH(e, heatbath(e->stream, ugeo(e, LGT_K_HEATBATH, mu), e->cur, mu, p, beta, e->seed,
+                                            e->sweep, ap, fold_projection));
+            }));
+        }
+    for (int k = 0; k < n_or; k++)
+        for (int mu = 0; mu < 4; mu++)
+            for (int p = 0; p < 2; p++)
+                TRY(run(e, LGT_K_OVERRELAX, "overrelax",
+                        [&] { return DISPATCH(e, overrelax(e->stream, ugeo(e, LGT_K_OVERRELAX, mu), e->cur, mu, p)); }));
+    if (n_or == 0 && !fold_projection)
+        TRY(run(e, LGT_K_PROJECT, "project", [&] { return DISPATCH(e, project(e->stream, e->geo, e->cur)); }));
+    return LGT_OK;
+}
+
 int alloc_field(lgt_engine* e, void** base, RawPlanes* P)
 {
     size_t bytes = (size_t)4 * e->vol * e->link_bytes, have = 0;
@@ -851,8 +916,8 @@ int lgt_metropolis_update(lgt_engine* e, int nsweeps, double beta, double eps, d
         return fail(LGT_EINVAL, "lgt_metropolis_update: beta = %g, eps = %g", beta, eps);
     TRY(set_dev(e));
     const uint32_t nb = nbx(e);
-    double accepted = 0.0;
-    if (acceptance) TRY(ensure(e, &e->partial, &e->partial_bytes, sizeof(double) * 2 * 8 * (size_t)nb, "partials"));
+    std::vector<double> acc_rows;
+    if (acceptance) TRY(acc_prepare(e, nsweeps, acc_rows));
     for (int s = 0; s < nsweeps; s++) {
         for (int mu = 0; mu < 4; mu++)
             for (int p = 0; p < 2; p++) {
@@ -862,15 +927,16 @@ int lgt_metropolis_update(lgt_engine* e, int nsweeps, double beta, double eps, d
                                                   e->seed, e->sweep, ap));
                 }));
             }
-        if (acceptance) {
-            double a = 0.0;
-            TRY(reduce_to_host(e, 1, (int)(8 * nb), &a));
-            accepted += a;
-        }
+        if (acceptance) TRY(acc_reduce(e, s));
         e->sweep++;
+        if (acceptance) TRY(acc_collect(e, s, nsweeps, acc_rows));
     }
     TRY(run(e, LGT_K_PROJECT, "project", [&] { return DISPATCH(e, project(e->stream, e->geo, e->cur)); }));
-    if (acceptance) *acceptance = nsweeps > 0 ? accepted / ((double)(4 * e->vol) * nsweeps) : 0.0;
+    if (acceptance) {
+        double accepted = 0.0;
+        for (double v : acc_rows) accepted += v;
+        *acceptance = nsweeps > 0 ? accepted / ((double)(4 * e->vol) * nsweeps) : 0.0;
+    }
     HIP_TRY(hipStreamSynchronize(e->stream), "hipStreamSynchronize");
     return LGT_OK;
 }
@@ -937,36 +1003,13 @@ int lgt_update(lgt_engine* e, int nsweeps, int n_or, double beta, double* accept
     if (!e || nsweeps < 0 || n_or < 0) return fail(LGT_EINVAL, "lgt_update: nsweeps/n_or must be >= 0");
     if (!(beta >= 0.0) || !std::isfinite(beta)) return fail(LGT_EINVAL, "lgt_update: beta = %g", beta);
     TRY(set_dev(e));
-    const uint32_t nb = nbx(e);
     std::vector<double> acc_rows;
-    if (acceptance) {
-        TRY(ensure(e, &e->partial, &e->partial_bytes, sizeof(double) * 2 * 8 * (size_t)nb, "partials"));
-        acc_rows.resize(nsweeps);
-    }
-    // A heatbath-only sweep ends with a projection of the whole field
-    // (update.c:102-176).  In FP64 it is a pass of its own, so that the staples
-    // of the sweep read the heatbath's unprojected links as the reference's
-    // do; in FP32 the heatbath stores its links projected instead, which
-    // saves that pass (one read and one write of the field).
-    const bool fold_projection = n_or == 0 && e->prec == LGT_FP32;
+    if (acceptance) TRY(acc_prepare(e, nsweeps, acc_rows));
     for (int s = 0; s < nsweeps; s++) {
-        for (int mu = 0; mu < 4; mu++)
-            for (int p = 0; p < 2; p++) {
-                void* ap = acceptance ? (char*)e->partial + sizeof(double) * 2 * (size_t)nb * (mu * 2 + p) : nullptr;
-                TRY(run(e, LGT_K_HEATBATH, "heatbath", [&] {
-                    return DISPATCH(e, heatbath(e->stream, ugeo(e, LGT_K_HEATBATH, mu), e->cur, mu, p, beta, e->seed,
-                                                e->sweep, ap, fold_projection));
-                }));
-            }
-        if (acceptance) TRY(reduce_to_host(e, 1, (int)(8 * nb), &acc_rows[s]));
-        for (int k = 0; k < n_or; k++)
-            for (int mu = 0; mu < 4; mu++)
-                for (int p = 0; p < 2; p++)
-                    TRY(run(e, LGT_K_OVERRELAX, "overrelax",
-                            [&] { return DISPATCH(e, overrelax(e->stream, ugeo(e, LGT_K_OVERRELAX, mu), e->cur, mu, p)); }));
-        if (n_or == 0 && !fold_projection)
-            TRY(run(e, LGT_K_PROJECT, "project", [&] { return DISPATCH(e, project(e->stream, e->geo, e->cur)); }));
+        TRY(enqueue_sweep(e, n_or, beta, acceptance != nullptr));
+        if (acceptance) TRY(acc_reduce(e, s));
         e->sweep++;
+        if (acceptance) TRY(acc_collect(e, s, nsweeps, acc_rows));
     }
     if (acceptance) {
         double a = 0.0;

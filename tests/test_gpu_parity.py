@@ -513,6 +513,26 @@ def test_fp32_update_stays_unitary(lgt):
     assert 0.3 < p < 0.8
 
 
+def test_acceptance_over_many_sweeps(lgt):
+    """The per-sweep acceptance is read back in batches of 1024 sweeps: one
+    call of 1030 sweeps gives the field of 1030 calls of one sweep bit for bit
+    and the mean of their acceptances; Metropolis sweeps give the same field
+    with and without acceptance."""
+    L = (4, 4, 4, 4)
+    n = 1030
+    with engine(lgt, L, 1, seed=21) as a, engine(lgt, L, 1, seed=21) as b:
+        a.init_field(1)
+        b.init_field(1)
+        acc = a.update(n, 1, 3.09, acceptance=True)
+        each = [b.update(1, 1, 3.09, acceptance=True) for _ in range(n)]
+        assert acc == pytest.approx(np.mean(each), rel=1e-12)
+        np.testing.assert_array_equal(a.download(), b.download())
+        acc = a.metropolis_update(n, 3.09, 0.5, acceptance=True)
+        b.metropolis_update(n, 3.09, 0.5)
+        np.testing.assert_array_equal(a.download(), b.download())
+        assert 0.3 < acc < 1.0
+
+
 def test_fp32_heatbath_only_projects_on_store(lgt):
     """An FP32 heatbath-only sweep stores every link projected onto SU(2)
     instead of projecting the field after the sweep: it equals a replay of
