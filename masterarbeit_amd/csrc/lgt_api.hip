@@ -403,16 +403,15 @@ int reduce_to_host(lgt_engine* e, int nrows, int ncols, double* out)
 // Per-sweep acceptance of the update loops: the 8 nb block partials of sweep
 // s are reduced on the stream into the double-double dres[s mod ACC_BATCH],
 // and the host reads a batch of those at once instead of waiting for every
-// sweep.  rows[s] = hi + lo of sweep s, as reduce_to_host gives it.
+// sweep.
 constexpr int ACC_BATCH = 1024;
 
-int acc_prepare(lgt_engine* e, int nsweeps, std::vector<double>& rows)
+int acc_prepare(lgt_engine* e, int nsweeps)
 {
     const size_t batch = (size_t)std::min(std::max(nsweeps, 1), ACC_BATCH);
     TRY(ensure(e, &e->partial, &e->partial_bytes, sizeof(double) * 2 * 8 * (size_t)nbx(e), "partials"));
     TRY(ensure(e, &e->dres, &e->dres_bytes, sizeof(double) * 2 * batch, "results"));
     TRY(ensure_host(e, sizeof(double) * 2 * batch));
-    rows.assign(nsweeps, 0.0);
     return LGT_OK;
 }
 
@@ -424,15 +423,18 @@ int acc_reduce(lgt_engine* e, int s)
     });
 }
 
-// at the end of sweep s: reads the batch back when it is full or s is the last
-int acc_collect(lgt_engine* e, int s, int nsweeps, std::vector<double>& rows)
+// at the end of sweep s: when the batch is full or s is the last sweep, reads
+// it back and hands each of its sweeps' sums (hi + lo, as reduce_to_host
+// rounds them) to add, in sweep order
+template <class F>
+int acc_collect(lgt_engine* e, int s, int nsweeps, F&& add)
 {
     if ((s + 1) % ACC_BATCH != 0 && s + 1 != nsweeps) return LGT_OK;
-    const int s0 = s - s % ACC_BATCH, n = s + 1 - s0;
+    const int n = s % ACC_BATCH + 1;
     HIP_TRY(hipMemcpyAsync(e->hres, e->dres, sizeof(double) * 2 * n, hipMemcpyDeviceToHost, e->stream),
             "hipMemcpyAsync");
     HIP_TRY(hipStreamSynchronize(e->stream), "hipStreamSynchronize");
-    for (int j = 0; j < n; j++) rows[s0 + j] = e->hres[2 * j] + e->hres[2 * j + 1];
+    for (int j = 0; j < n; j++) add(e->hres[2 * j] + e->hres[2 * j + 1]);
     return LGT_OK;
 }
 
@@ -916,8 +918,8 @@ int lgt_metropolis_update(lgt_engine* e, int nsweeps, double beta, double eps, d
         return fail(LGT_EINVAL, "lgt_metropolis_update: beta = %g, eps = %g", beta, eps);
     TRY(set_dev(e));
     const uint32_t nb = nbx(e);
-    std::vector<double> acc_rows;
-    if (acceptance) TRY(acc_prepare(e, nsweeps, acc_rows));
+    double accepted = 0.0;
+    if (acceptance) TRY(acc_prepare(e, nsweeps));
     for (int s = 0; s < nsweeps; s++) {
         for (int mu = 0; mu < 4; mu++)
             for (int p = 0; p < 2; p++) {
@@ -929,14 +931,10 @@ int lgt_metropolis_update(lgt_engine* e, int nsweeps, double beta, double eps, d
             }
         if (acceptance) TRY(acc_reduce(e, s));
         e->sweep++;
-        if (acceptance) TRY(acc_collect(e, s, nsweeps, acc_rows));
+        if (acceptance) TRY(acc_collect(e, s, nsweeps, [&](double v) { accepted += v; }));
     }
     TRY(run(e, LGT_K_PROJECT, "project", [&] { return DISPATCH(e, project(e->stream, e->geo, e->cur)); }));
-    if (acceptance) {
-        double accepted = 0.0;
-        for (double v : acc_rows) accepted += v;
-        *acceptance = nsweeps > 0 ? accepted / ((double)(4 * e->vol) * nsweeps) : 0.0;
-    }
+    if (acceptance) *acceptance = nsweeps > 0 ? accepted / ((double)(4 * e->vol) * nsweeps) : 0.0;
     HIP_TRY(hipStreamSynchronize(e->stream), "hipStreamSynchronize");
     return LGT_OK;
 }
@@ -1003,19 +1001,15 @@ int lgt_update(lgt_engine* e, int nsweeps, int n_or, double beta, double* accept
     if (!e || nsweeps < 0 || n_or < 0) return fail(LGT_EINVAL, "lgt_update: nsweeps/n_or must be >= 0");
     if (!(beta >= 0.0) || !std::isfinite(beta)) return fail(LGT_EINVAL, "lgt_update: beta = %g", beta);
     TRY(set_dev(e));
-    std::vector<double> acc_rows;
-    if (acceptance) TRY(acc_prepare(e, nsweeps, acc_rows));
+    double a = 0.0;
+    if (acceptance) TRY(acc_prepare(e, nsweeps));
     for (int s = 0; s < nsweeps; s++) {
         TRY(enqueue_sweep(e, n_or, beta, acceptance != nullptr));
         if (acceptance) TRY(acc_reduce(e, s));
         e->sweep++;
-        if (acceptance) TRY(acc_collect(e, s, nsweeps, acc_rows));
+        if (acceptance) TRY(acc_collect(e, s, nsweeps, [&](double v) { a += v / (double)(4 * e->vol); }));
     }
-    if (acceptance) {
-        double a = 0.0;
-        for (double v : acc_rows) a += v / (double)(4 * e->vol);
-        *acceptance = nsweeps > 0 ? a / nsweeps : 0.0;
-    }
+    if (acceptance) *acceptance = nsweeps > 0 ? a / nsweeps : 0.0;
     HIP_TRY(hipStreamSynchronize(e->stream), "hipStreamSynchronize");
     return LGT_OK;
 }
