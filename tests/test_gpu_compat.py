@@ -15,7 +15,7 @@ pytestmark = pytest.mark.gpu
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 DRIVER = os.path.join(ROOT, "tests", "compat", "_build", "compat_driver")
 DRIVER_LAZY = os.path.join(ROOT, "tests", "compat", "_build", "compat_driver_lazy")
-SEED = ((1 << 30) << 32) | (1 << 31)     # drawn from the driver's ranlxd: 0.25, 0.5
+SEED = ((1 << 30) << 32) | (1 << 31)     # drawn from compat_driver's ranlxd: 0.25, 0.5
 
 
 def read_summary(path):
