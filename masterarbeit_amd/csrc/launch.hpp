@@ -30,7 +30,7 @@ template <typename T> struct Launch {
     static hipError_t gauge(hipStream_t s, const Geo& g, const RawPlanes& P, uint64_t seed);
     static hipError_t gauge_elements(hipStream_t s, const Geo& g, uint64_t seed, double* out);
     // acc_partial: nullable, receives ceil(vh/256) block sums of 1/count;
-    // project: store the new links projected onto SU(2)
+    // project (FP32 only): store the new links projected onto SU(2)
     static hipError_t heatbath(hipStream_t s, const Geo& g, const RawPlanes& P, int mu, int parity, double beta,
                                uint64_t seed, uint32_t sweep, void* acc_partial, bool project = false);
     static hipError_t overrelax(hipStream_t s, const Geo& g, const RawPlanes& P, int mu, int parity);

@@ -73,15 +73,24 @@ hipError_t Launch<T>::heatbath(hipStream_t s, const Geo& g, const RawPlanes& P, 
     Planes<T> Pl = planes<T>(P);
     DD* ap = (DD*)acc_partial;
     auto go = [&](auto kernel) { hipLaunchKernelGGL(kernel, grid, block, 0, s, Pl, g, parity, (T)beta, seed, sweep, ap); };
-    switch (mu * 2 + (project ? 1 : 0)) {
+    if (project) {
+        // only FP32 fields fold the projection into the heatbath
+        if constexpr (sizeof(T) == 4) {
+            switch (mu) {
+            case 0: go(k_heatbath<T, 0, true>); break;
+            case 1: go(k_heatbath<T, 1, true>); break;
+            case 2: go(k_heatbath<T, 2, true>); break;
+            default: go(k_heatbath<T, 3, true>); break;
+            }
+            return hipGetLastError();
+        }
+        return hipErrorInvalidValue;
+    }
+    switch (mu) {
     case 0: go(k_heatbath<T, 0, false>); break;
-    case 1: go(k_heatbath<T, 0, true>); break;
-    case 2: go(k_heatbath<T, 1, false>); break;
-    case 3: go(k_heatbath<T, 1, true>); break;
-    case 4: go(k_heatbath<T, 2, false>); break;
-    case 5: go(k_heatbath<T, 2, true>); break;
-    case 6: go(k_heatbath<T, 3, false>); break;
-    default: go(k_heatbath<T, 3, true>); break;
+    case 1: go(k_heatbath<T, 1, false>); break;
+    case 2: go(k_heatbath<T, 2, false>); break;
+    default: go(k_heatbath<T, 3, false>); break;
     }
     return hipGetLastError();
 }
