@@ -222,7 +222,8 @@ int lgt_ape(lgt_engine *e, int mode, int nsteps, double sp);
 /* measureWilsonLoop(w) (LGT_Code/modules/obs/wilson.c:268-324) with the loop
  * list given explicitly (measParams.rExtents/tExtents, wilson.c:274-285):
  * w[i] = mean over sites x and spatial d of (1/2) Re tr W_{R[i] x T[i]}(x; t, d).
- * Requires 1 <= T[i] <= extents[0]-1 (wilson.c:290-291) and R[i] >= 1.
+ * Requires 1 <= T[i] <= extents[0]-1 (wilson.c:290-291) and 1 <= R[i] <= 4096
+ * (a spatial line longer than the lattice wraps around it).
  * Device memory: the smearing buffer (one field) holds the spatial lines; a
  * list with more than one R also allocates a second line field (three
  * directions), kept for later calls. */
@@ -259,7 +260,8 @@ int lgt_ape_vjp_forward(lgt_engine *e, int mode, int nsteps, double sp, int segm
 int lgt_ape_vjp_backward(lgt_engine *e, const double *cotangent, double *grad);
 /* The R x T loops of measureWilsonLoop (LGT_Code/modules/obs/wilson.c:21-56)
  * summed without normalisation, L = sum_x sum_{d=1..3} 2 Re[W(x; t, d)]_0,
- * returned as an unevaluated double-double sum[0] + sum[1]. */
+ * returned as an unevaluated double-double sum[0] + sum[1].  R and T as in
+ * lgt_wilson_loops. */
 int lgt_wloop_sum(lgt_engine *e, int R, int T, double sum[2]);
 /* dL/dU of that L for every link of the current field; kept on the device as
  * the pending cotangent of lgt_ape_vjp_backward, and copied to grad (field

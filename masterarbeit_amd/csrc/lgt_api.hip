@@ -559,8 +559,8 @@ int copy_dirs(lgt_engine* e, const RawPlanes& dst, const RawPlanes& src, DirRang
 
 int check_loop(const lgt_engine* e, int R, int T)
 {
-    if (T < 1 || T > e->L[0] - 1 || T > WMAXT_HOST || R < 1)
-        return fail(LGT_EINVAL, "loop R=%d T=%d: need R >= 1 and 1 <= T <= %d", R, T, e->L[0] - 1);
+    if (T < 1 || T > e->L[0] - 1 || T > WMAXT_HOST || R < 1 || R > 4096)
+        return fail(LGT_EINVAL, "loop R=%d T=%d: need 1 <= R <= 4096 and 1 <= T <= %d", R, T, e->L[0] - 1);
     return LGT_OK;
 }
 
@@ -1149,7 +1149,7 @@ int lgt_wilson_loops(lgt_engine* e, int nloop, const int* R, const int* T, doubl
     for (int i = 0; i < nloop; i++) {
         if (T[i] < 1 || T[i] > e->L[0] - 1 || T[i] > WMAXT_HOST)
             return fail(LGT_EINVAL, "lgt_wilson_loops: T = %d outside [1, %d] (lattice too small)", T[i], e->L[0] - 1);
-        if (R[i] < 1) return fail(LGT_EINVAL, "lgt_wilson_loops: R = %d < 1", R[i]);
+        if (R[i] < 1 || R[i] > 4096) return fail(LGT_EINVAL, "lgt_wilson_loops: R = %d outside [1, 4096]", R[i]);
     }
     TRY(set_dev(e));
     TRY(ensure_alt(e));

@@ -226,6 +226,10 @@ def test_wilson_loop_rejects_long_t(lgt):
     with engine(lgt, (4, 4, 4, 4), 0) as e:
         with pytest.raises(lgt.LgtError):
             e.wilson_loops([1], [4])                       # wilson.c:290-291: T <= L0 - 1
+        for R in (0, 4097, 1 << 30):                       # spatial lines of 1..4096 links
+            with pytest.raises(lgt.LgtError):
+                e.wilson_loops([1, R], [1, 1])
+        assert np.isfinite(e.wilson_loops([4096], [1])[0])
 
 
 @pytest.mark.parametrize("L", LATS)
