@@ -1092,8 +1092,9 @@ __device__ __forceinline__ DD wave_sum8_dd(const double (&v)[8], int lane)
 // (wave_sum8_dd).
 // grid: x over (tile, parity) pairs, y = d - 1; WL<T>::THREADS per block
 template <typename T> struct WL {
-    // sites per lane; 48^4, 160 loops with the batched wave sums:
-    // FP32 1, 2, 4 sites -> 9.5, 7.9, 8.5 ms; FP64 -> 16.2, 15.6, 17.7 ms
+    // sites per lane; 48^4, 160 loops with the batched wave sums and one
+    // carried product per site: FP32 1, 2, 4 sites -> 9.0, 7.0, 8.4 ms;
+    // FP64 -> 13.1, 13.2, 18.9 ms
     static constexpr int SITES = 2;
     static constexpr int THREADS = BLOCK / SITES;
     static constexpr int NW = THREADS / 64;
@@ -1118,7 +1119,10 @@ __global__ void __launch_bounds__(WL<T>::THREADS) k_wloops(Planes<T> P, Planes<T
     // index, advanced one time step per T (t0: their time offset, mod L0)
     int tT[NS], tR[NS];
     uint32_t cbT[NS], cbR[NS];
-    Q<T> Lx[NS], B[NS];   // L_T(x) and S_R(x) L_T(x + R d)
+    // M = B^+ L_T(x) with B = S_R(x) L_T(x + R d): the loop is
+    // L_T(x) S_R(x + T t) B^+, so Re tr(loop) = Re tr(M S_R(x + T t)), and one
+    // step in T is M <- U_0(x + R d + T t)^+ M U_0(x + T t)
+    Q<T> M[NS];
     bool valid[NS];
     const int pR = p ^ (R & 1);   // parity of x + R d
     const uint32_t st0 = g.stride[0], wrap0 = (uint32_t)(g.L[0] - 1) * st0;
@@ -1133,9 +1137,8 @@ __global__ void __launch_bounds__(WL<T>::THREADS) k_wloops(Planes<T> P, Planes<T
         cbT[j] = valid[j] ? i : 0u;
         tR[j] = sR.c[0];
         cbR[j] = cb_index(g, sR);
-        Lx[j] = link(P, g, 0, s0);
         const Q<T> S0 = link(S, g, d, s0);
-        B[j] = qmul(S0, link(P, g, 0, sR));
+        M[j] = qmul_c(qdag(qmul(S0, link(P, g, 0, sR))), link(P, g, 0, s0));
         if (Rn > 0 && valid[j]) {
             // the next group's line S_Rn(x) = S_R(x) U_d(x+Rd) ... U_d(x+(Rn-1)d),
             // the products in k_sline's order
@@ -1164,8 +1167,8 @@ __global__ void __launch_bounds__(WL<T>::THREADS) k_wloops(Planes<T> P, Planes<T
 #pragma unroll
             for (int j = 0; j < NS; j++) {
                 if (Tt > 1) {
-                    Lx[j] = qmul_c(Lx[j], link_at(P, 0, (((uint32_t)p ^ odd) * g.vh + cbT[j]) * LB));
-                    B[j] = qmul_c(B[j], link_at(P, 0, (((uint32_t)pR ^ odd) * g.vh + cbR[j]) * LB));
+                    const Q<T> Ub = link_at(P, 0, (((uint32_t)pR ^ odd) * g.vh + cbR[j]) * LB);
+                    M[j] = qmul_c(qmul_c(qdag(Ub), M[j]), link_at(P, 0, (((uint32_t)p ^ odd) * g.vh + cbT[j]) * LB));
                 }
                 // one step in t
                 cbT[j] += (tT[j] + 1 == g.L[0]) ? (uint32_t)0 - wrap0 : st0;
@@ -1174,7 +1177,7 @@ __global__ void __launch_bounds__(WL<T>::THREADS) k_wloops(Planes<T> P, Planes<T
                 tR[j] = (tR[j] + 1 == g.L[0]) ? 0 : tR[j] + 1;
                 if (slot >= 0) {
                     const Q<T> St = link_at(S, d, (((uint32_t)p ^ odd ^ 1u) * g.vh + cbT[j]) * LB);
-                    const T w = T(2) * qdot_c(qmul_c(Lx[j], St), B[j]);
+                    const T w = T(2) * qdot_c(M[j], qdag(St));   // 2 Re[M St]_0
                     acc[k] += valid[j] ? (double)w : 0.0;
                 }
             }
