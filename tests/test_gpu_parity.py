@@ -292,6 +292,51 @@ def test_heatbath_links_match_reference(lgt):
     assert np.max(np.abs(links - fx["links"])) <= 1e-14
 
 
+@pytest.mark.parametrize("prec", [0, 1])
+def test_heatbath_draw_distribution(lgt, prec):
+    """On a cold field every staple is 6 * 1, so a heatbath sub-sweep draws
+    each new link straight from the Kennedy-Pendleton density at alpha = 6 beta
+    (heatbath.c:76-123): at beta = 7/6, x0 = U.c0 of 663 552 links (four
+    sub-sweeps of a 24^4 lattice) follows p(x0) ~ sqrt(1 - x0^2) exp(7 x0) and
+    the reference's own 10^8-draw histogram (LGT_Code/main_test/improved.dat)
+    by chi^2, and the direction of (c1, c2, c3) is isotropic, in both storages
+    (FP32: hardware log2/cos and 24-bit uniforms)."""
+    L = (24, 24, 24, 24)
+    alpha = 7.0
+    even = F.coords(L).sum(axis=0) % 2 == 0
+    links = []
+    with engine(lgt, L, prec, seed=77) as e:
+        for k in range(4):
+            e.init_field(0)
+            e.sweep_index = k
+            e.heatbath_subsweep(0, 0, alpha / 6.0)
+            links.append(e.download()[even, 0])
+    links = np.concatenate(links)
+    n = len(links)
+    np.testing.assert_allclose(np.linalg.norm(links, axis=1), 1.0, atol=1e-15 if prec == 0 else 3e-7)
+    edges = np.linspace(-1, 1, 201)
+    hist, _ = np.histogram(links[:, 0], bins=edges)
+    xs = np.linspace(-1, 1, 200001)
+    dens = np.sqrt(1 - xs ** 2) * np.exp(alpha * xs)
+    cdf = np.concatenate([[0], np.cumsum((dens[1:] + dens[:-1]) / 2)])
+    cdf /= cdf[-1]
+    expect = n * np.diff(np.interp(edges, xs, cdf))
+    sel = expect > 50
+    chi2 = (((hist - expect) ** 2) / expect)[sel].sum()
+    dof = sel.sum()
+    ref = np.loadtxt(os.path.join(GOLDEN, "kp_improved_alpha7.dat"))
+    ref_counts = ref[:, 1] / 200.0 * n                    # value = count * M / N
+    selr = ref_counts > 50
+    chi2r = (((hist - ref_counts) ** 2) / ref_counts)[selr].sum()
+    v = links[:, 1:] / np.linalg.norm(links[:, 1:], axis=1, keepdims=True)
+    print(f"prec {prec}: chi2 {chi2:.1f} / {dof} bins, against the reference histogram {chi2r:.1f} / "
+          f"{selr.sum()}, direction mean {np.abs(v.mean(axis=0)).max():.1e}")
+    assert chi2 < dof + 5 * np.sqrt(2 * dof), (chi2, dof)
+    assert chi2r < selr.sum() + 5 * np.sqrt(2 * selr.sum()), chi2r
+    np.testing.assert_allclose(v.mean(axis=0), 0.0, atol=0.01)
+    np.testing.assert_allclose((v ** 2).mean(axis=0), 1.0 / 3.0, atol=0.01)
+
+
 def test_heatbath_degenerate_branch(lgt):
     fx = load("su2_random.npz")
     n = len(fx["links"])
