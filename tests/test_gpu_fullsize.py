@@ -112,7 +112,7 @@ def test_observables_are_gauge_invariant(lgt, thermal):
     assert abs(c1[0] - c0[0]) <= (1e-12 if prec == 0 else 1e-8) * c0[1]
 
 
-@pytest.mark.parametrize("mode,sp", [(0, 0.25), (1, 1.0 / 6.0)], ids=["spatial", "4d"])
+@pytest.mark.parametrize("mode,sp", [(0, 0.25), (1, 1.0 / 6.0), (2, 0.25)], ids=["spatial", "4d", "temporal"])
 def test_smearing_is_gauge_covariant(lgt, thermal, mode, sp):
     """Smearing the gauge transform of the 48^4 field gives the gauge transform
     of the smeared field (FP32 spatial smearing through the site-pair kernel)."""
