@@ -668,14 +668,16 @@ def test_metropolis_update_matches_oracle(lgt, prec):
         assert abs(acc - acc_ref) < 0.02
 
 
-def test_metropolis_equilibrium_matches_heatbath(lgt):
+@pytest.mark.parametrize("prec", [0, 1])
+def test_metropolis_equilibrium_matches_heatbath(lgt, prec):
     """Metropolis and heatbath chains sample the same Wilson action: their
     mean plaquettes at beta = 3.09 agree within 4 sigma (8^4, independent
-    chains of 600 measured sweeps after 200 of thermalisation)."""
+    chains of 600 measured sweeps after 200 of thermalisation), in both
+    storages."""
     L = (8, 8, 8, 8)
     means = []
     for algo in ("hb", "metro"):
-        with engine(lgt, L, 0, seed=31 if algo == "hb" else 32) as e:
+        with engine(lgt, L, prec, seed=31 if algo == "hb" else 32) as e:
             e.init_field(1)
             p = []
             for k in range(800):
