@@ -1007,12 +1007,12 @@ __global__ void __launch_bounds__(BLOCK) k_ape_zpair(Planes<T> Z, Planes<T> O, G
 // Wilson loops W(R,T) in the (t, d) planes, d = 1..3 (LGT_Code/modules/obs/
 // wilson.c:21-56,268-324), fused over all T of one R:
 //   W(x) = Re tr[ L_T(x) S_R(x+T t) L_T(x+R d)^+ S_R(x)^+ ]
-//        = 2 < L_T(x) S_R(x+T t), S_R(x) L_T(x+R d) >
-// with the temporal line L_T(x) and the product S_R(x) L_T(x+R d) each grown
-// by one link per step (three products per T), and the spatial lines S_R
-// read from a line field built beforehand (k_sline), one load per step
-// instead of R.  Results per (group, d, parity, tile) are reduced in
-// double-double; every wave adds its shuffle-reduced value to its own LDS slot.
+//        = Re tr[ M S_R(x+T t) ],  M = L_T(x+R d)^+ S_R(x)^+ L_T(x)
+// with M carried from T to T by one link on each side (two products per T),
+// and the spatial lines S_R read from a line field built beforehand
+// (k_sline), one load per step instead of R.  Results per (group, d, parity,
+// tile) are reduced in double-double; every wave adds its shuffle-reduced
+// value to its own LDS slot.
 // ---------------------------------------------------------------------------
 struct WGroup { int R, Tmax, slot_off, pad; };
 constexpr int WMAXT = 128;

@@ -81,7 +81,7 @@ __device__ __forceinline__ Q<T> qmul_dag(const Q<T>& v, const Q<T>& w)
 }
 
 // Products for code with no reference rounding to reproduce (the adjoint of
-// the smearing, the Wilson loops' grown lines): FMA contraction in either
+// the smearing, the Wilson loops' carried product): FMA contraction in either
 // precision.  v*w, v*w^+, <a,b>, and a + v*w, a + v*w^+ with the sum in each
 // component's FMA chain.
 template <typename T>
